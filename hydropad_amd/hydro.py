@@ -1,0 +1,182 @@
+"""Host-side mirror of Hydropad's hydro-step interface over the C-ABI.
+
+`Box` owns one device-resident periodic box (an hp_ctx).  Its methods carry
+the reference's subroutine names and argument meaning:
+
+    ppmsolver(ndirection)   hydro/ppmsolver.F90:1-5      one sweep, in place
+    ppm_lxlylz(nrot)        hydro/ppm_lxlylz.F90:1-223   three sweeps in nrot order
+    exchange_mesh(dir=0)    hydropad/exchange_mesh.F90   periodic ghost fill
+    speed()                 hydropad/speed.F90:2-59      CFL maximum velocity
+    timedet(velmax, cour)   hydropad/functions.F90:2-35  next dt
+    evolve(first, n, cour)  hydropad/main.F90:100-121 + evolve_sys.F90
+
+Fields are Fortran-ordered float64 arrays of shape (nx, ny, nz) including
+nbound=3 ghost cells per side, as the reference stores p3d/rho3d/vx3d/vy3d/vz3d.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import HP_FP64, HP_MIXED, HpCounters, HpParams, check
+
+FIELDS = ("p", "rho", "vx", "vy", "vz")
+NBOUND = 3
+PRECISIONS = {"fp64": HP_FP64, "mixed": HP_MIXED}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def nrot_of_step(nstep):
+    """Sweep-order index of step nstep (hydropad/evolve_sys.F90:30-31)."""
+    return 1 if nstep == 1 else nstep % 6 + 1
+
+
+NROT_ORDER = {1: (1, 2, 3), 2: (3, 2, 1), 3: (2, 3, 1), 4: (1, 3, 2), 5: (3, 1, 2), 6: (2, 1, 3)}
+
+
+def timedet(params, velmax, cour, tstop=1e30, t=0.0):
+    """Non-cosmological timedet (hydropad/functions.F90:2-35) on the host.
+
+    Returns (dt, t_new) and updates params.dt / params.rdtath in place; the
+    arithmetic is the reference's, in IEEE double."""
+    told = t
+    rdtc = velmax / (cour * params.dx)
+    dt = 0.1 if velmax == 0.0 else 1.0 / rdtc
+    t = told + dt
+    if t > tstop:
+        t = tstop
+        dt = t - told
+    params.at, params.dat, params.rat = 1.0, 0.0, 1.0
+    params.dt = dt
+    params.rdtath = dt / 1.0
+    return dt, t
+
+
+class Box:
+    """One periodic box resident on the GPU."""
+
+    def __init__(self, shape, precision="fp64", gamma=5.0 / 3.0, nbound=NBOUND):
+        if isinstance(shape, int):
+            shape = (shape + 2 * nbound,) * 3
+        self.shape = tuple(int(s) for s in shape)
+        self.nbound = nbound
+        self.interior = tuple(s - 2 * nbound for s in self.shape)
+        self.L = _capi.lib()
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        h = ctypes.c_void_p()
+        check("hp_create", self.L.hp_create(*self.shape, nbound, prec, ctypes.byref(h)))
+        self._h = h
+        self.precision = precision
+        self.params = _capi.params_init(gamma, max(self.interior), 0.0)
+        self.set_params(self.params)
+
+    # ---- lifetime ----
+    def close(self):
+        if self._h:
+            self.L.hp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- state ----
+    def upload(self, state, gforce=None):
+        arrs = []
+        for k in FIELDS:
+            a = state[k]
+            if a.shape != self.shape or a.dtype != np.float64 or not a.flags.f_contiguous:
+                raise ValueError(f"field {k} must be float64 Fortran-ordered {self.shape}")
+            arrs.append(a)
+        g = None
+        if gforce is not None:
+            g = np.asfortranarray(gforce, dtype=np.float64)
+            if g.shape != (3,) + self.shape:
+                raise ValueError("gforce must be (3, nx, ny, nz)")
+        check("hp_upload", self.L.hp_upload(self._h, *[_ptr(a) for a in arrs],
+                                             _ptr(g) if g is not None else None))
+
+    def download(self):
+        out = {k: np.empty(self.shape, dtype=np.float64, order="F") for k in FIELDS}
+        check("hp_download", self.L.hp_download(self._h, *[_ptr(out[k]) for k in FIELDS]))
+        return out
+
+    def set_params(self, params):
+        self.params = params
+        check("hp_set_params", self.L.hp_set_params(self._h, ctypes.byref(params)))
+
+    def set_precision(self, precision):
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check("hp_set_precision", self.L.hp_set_precision(self._h, prec))
+        self.precision = precision
+
+    # ---- the reference's subroutines ----
+    def ppmsolver(self, ndirection):
+        check("hp_ppmsolver", self.L.hp_ppmsolver(self._h, int(ndirection)))
+
+    def exchange_mesh(self, direction=0):
+        check("hp_exchange_mesh", self.L.hp_exchange_mesh(self._h, int(direction)))
+
+    def ppm_lxlylz(self, nrot):
+        check("hp_ppm_lxlylz", self.L.hp_ppm_lxlylz(self._h, int(nrot)))
+
+    def speed(self):
+        v = ctypes.c_double()
+        check("hp_speed", self.L.hp_speed(self._h, ctypes.byref(v)))
+        return v.value
+
+    def timedet(self, velmax, cour, tstop=1e30, t=0.0):
+        dt, t = timedet(self.params, velmax, cour, tstop, t)
+        self.set_params(self.params)
+        return dt, t
+
+    def evolve(self, first_step, nsteps, cour, t, dt, tstop=1e30):
+        """Device-resident time loop; returns (t, dt) after the last step."""
+        tt, dd = ctypes.c_double(t), ctypes.c_double(dt)
+        check("hp_evolve", self.L.hp_evolve(self._h, int(first_step), int(nsteps), cour, tstop,
+                                            ctypes.byref(tt), ctypes.byref(dd)))
+        self.params.dt = dd.value
+        self.params.rdtath = dd.value
+        return tt.value, dd.value
+
+    def counters(self):
+        c = HpCounters()
+        rc = self.L.hp_status(self._h, ctypes.byref(c))
+        if rc not in (_capi.HP_OK, _capi.HP_E_NEWTON):
+            check("hp_status", rc)
+        return c
+
+    def reset_counters(self):
+        check("hp_reset_counters", self.L.hp_reset_counters(self._h))
+
+    def sync(self):
+        check("hp_sync", self.L.hp_sync(self._h))
+
+
+def integrate_cell(params, fl10, st5, etot, eint, ghalf):
+    """hydro/integrate.F90 on one cell, run on the device; returns new (p,rho,vx,vy,vz)."""
+    L = _capi.lib()
+    a = [ctypes.c_double(float(v)) for v in list(fl10) + list(st5) + [etot, eint, ghalf]]
+    check("hp_integrate_cell", L.hp_integrate_cell(ctypes.byref(params), *[ctypes.byref(v) for v in a]))
+    return np.array([a[10 + i].value for i in range(5)])
+
+
+def dualenergy_cell(params, etotnew, rhonew, vx, vy, vz, eintnew_in, eint_old, pres_old):
+    """Dual-energy selection of hydro/integrate.F90:196-216, run on the device."""
+    L = _capi.lib()
+    pn, en = ctypes.c_double(), ctypes.c_double()
+    check("hp_dualenergy_cell", L.hp_dualenergy_cell(ctypes.byref(params), etotnew, rhonew, vx, vy, vz,
+                                                     eintnew_in, eint_old, pres_old, ctypes.byref(pn),
+                                                     ctypes.byref(en)))
+    return pn.value, en.value

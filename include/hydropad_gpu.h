@@ -1,0 +1,152 @@
+/*
+ * hydropad_gpu.h -- C-ABI of the MI355X-native directionally split PPM hydro step.
+ *
+ * This is the drop-in boundary for Hydropad's hydro hot path.  Every entry
+ * point names the reference interface it replaces (paths relative to the
+ * cgheller/hydropad tree).  All arguments are plain pointers, sizes and
+ * scalars; no C++ or framework types cross this boundary, so the header can be
+ * bound from Fortran (ISO_C_BINDING, see hydropad_amd/fortran/hydropad_gpu.F90),
+ * from C, or from Python ctypes.
+ *
+ * Conventions
+ *  - Fields are REAL(8) arrays in Fortran (column-major, x fastest) order with
+ *    dimensions (nx,ny,nz) = (N+2*nbound, ...), ghosts included, exactly as the
+ *    reference allocates p3d/rho3d/vx3d/vy3d/vz3d (hydropad/alloc_arrays.F90:30-32,56-60).
+ *  - gforce, when given, is REAL(8) (3,nx,ny,nz) interleaved (hydropad/alloc_arrays.F90:89).
+ *  - Every call returns 0 on success and a negative HP_E* code on failure; the
+ *    reference's `stop` paths (Newton non-convergence, hydro/riemann.F90:62-65)
+ *    become counters plus HP_E_NEWTON instead of terminating the process.
+ *  - A context owns its device state and one HIP stream; calls on one context
+ *    are stream-ordered and must come from one host thread at a time.
+ */
+#ifndef HYDROPAD_GPU_H
+#define HYDROPAD_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define HP_OK            0
+#define HP_E_ARG        -1   /* bad argument (null pointer, bad size, bad direction) */
+#define HP_E_HIP        -2   /* a HIP runtime call failed */
+#define HP_E_NOMEM      -3   /* device allocation failed / budget exceeded */
+#define HP_E_NEWTON     -4   /* Riemann Newton iteration exceeded 100 steps (hydro/riemann.F90:62-65) */
+#define HP_E_STATE      -5   /* call not valid in the context's current state */
+
+/* ---- precision forms ---------------------------------------------------- */
+#define HP_FP64          0   /* everything in FP64 */
+#define HP_MIXED         1   /* FP32 reconstruction/tracing/fluxes; FP64 Newton + dual energy + state */
+
+/*
+ * The 19 module scalars the reference pushes to the device before every sweep
+ * (hydropad/scalar.F90:98-99, hydro/ppmsolver.F90:68-69).  The gamma-derived
+ * members follow hydropad/initialize_parameters.F90:64-69, dx/rdx follow
+ * hydropad/initialize_parameters.F90:23-24 (dx is rounded through FP32 there).
+ */
+typedef struct hp_params {
+    double gamma, gamma1, rgamma, rgamma1, m, rm, gf;
+    double dt, dx, rdx, rdtath, at, dat, rat;
+    double flatvalue, mingradflat, eta1, eta2, dmax;
+} hp_params;
+
+/* Counters that replace the reference's prints and `stop`s
+ * (hydro/order2interp.F90:174-190, hydro/riemann.F90:62-65, hydro/integrate.F90:190-217). */
+typedef struct hp_counters {
+    int64_t newton_iters_max;   /* largest Newton iteration count seen */
+    int64_t newton_nonconv;     /* interfaces whose Newton loop hit 100 iterations */
+    int64_t neg_state_clamps;   /* traced p/rho < 0 clamped to 1e-15 */
+    int64_t reserved[5];
+} hp_counters;
+
+typedef struct hp_ctx hp_ctx;
+
+/* Fill the derived constants exactly as hydropad/initialize_parameters.F90:23-24,64-69
+ * and :183-203 (non-cosmological time init: at=1, dat=0, rat=1, rdtath=dt).
+ * ngrid_max is max(ngridx,ngridy,ngridz).  Host-only; never touches a GPU. */
+int hp_params_init(double gamma, int ngrid_max, double dt, hp_params* out);
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the OpenACC `declare create` of the module fields (hydropad/matrix.F90:24)
+ * and the per-sweep temporaries of hydro/ppmsolver.F90:55-64: device state is
+ * allocated once here and stays resident. precision is HP_FP64 or HP_MIXED. */
+int  hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out);
+void hp_destroy(hp_ctx* ctx);
+int  hp_set_precision(hp_ctx* ctx, int precision);
+
+/* Host<->device copies of the 5 fields (ghosts included) and the optional
+ * gravity array; replaces `!$acc update device/host` (hydro/ppmsolver.F90:95,
+ * hydro/ppm_lxlylz.F90:20).  gforce may be NULL (no gravity, g == 0). */
+int hp_upload(hp_ctx* ctx, const double* p, const double* rho, const double* vx,
+              const double* vy, const double* vz, const double* gforce);
+int hp_download(hp_ctx* ctx, double* p, double* rho, double* vx, double* vy, double* vz);
+
+/* Device-to-device variants: pointers are device (or managed) memory. */
+int hp_upload_device(hp_ctx* ctx, const double* p, const double* rho, const double* vx,
+                     const double* vy, const double* vz, const double* gforce);
+int hp_download_device(hp_ctx* ctx, double* p, double* rho, double* vx, double* vy, double* vz);
+
+/* The module scalars for the next sweep(s); replaces hydro/ppmsolver.F90:68-69. */
+int hp_set_params(hp_ctx* ctx, const hp_params* prm);
+
+/* ---- the hot path --------------------------------------------------------- */
+/* One directional sweep in place on the device state, ndirection 1=x 2=y 3=z.
+ * Replaces SUBROUTINE ppmsolver (hydro/ppmsolver.F90:1-467, STENCIL branch
+ * :94-160, :213-288, :341-416).  On return the face ghosts of all three
+ * directions are periodic-consistent, as after the reference's exchange_mesh. */
+int hp_ppmsolver(hp_ctx* ctx, int ndirection);
+
+/* Periodic ghost fill of the 5 fields on the device; replaces SUBROUTINE
+ * exchange_mesh (hydropad/exchange_mesh.F90:2-126, exchange_var.F90:2-57) on one
+ * rank.  direction 1..3 fills that axis' faces; 0 fills all, edges and corners
+ * included, in the order z, x, y of exchange_mesh.F90:47-123. */
+int hp_exchange_mesh(hp_ctx* ctx, int direction);
+
+/* One full hydro step: three sweeps in the order given by nrot (1..6) with the
+ * ghost fills in between; replaces SUBROUTINE ppm_lxlylz (hydro/ppm_lxlylz.F90:1-223). */
+int hp_ppm_lxlylz(hp_ctx* ctx, int nrot);
+
+/* CFL maximum velocity max(|v_d| + c_s) over interior cells, reduced on the
+ * device; replaces SUBROUTINE speed (hydropad/speed.F90:2-59) incl. its
+ * MPI_Allreduce.  Only 8 bytes cross PCIe. */
+int hp_speed(hp_ctx* ctx, double* velmax);
+
+/* Device-resident time loop shaped like hydropad/main.F90:100-121 +
+ * evolve_sys.F90:30-31,54,122-129: for nstep = first_step .. first_step+nsteps-1,
+ * nrot = mod(nstep,6)+1 (1 on step 1), ppm_lxlylz, speed, timedet
+ * (hydropad/functions.F90:2-35).  cour is the Courant number; dt/t are in/out.
+ * No host synchronisation happens inside the loop. */
+int hp_evolve(hp_ctx* ctx, int first_step, int nsteps, double cour, double tstop,
+              double* t, double* dt);
+
+/* Accumulated counters since the last hp_reset_counters. */
+int hp_status(hp_ctx* ctx, hp_counters* out);
+int hp_reset_counters(hp_ctx* ctx);
+int hp_sync(hp_ctx* ctx);
+
+/* ---- scalar entry points (host-callable, for unit vectors) ---------------- */
+/* SUBROUTINE integrate (hydro/integrate.F90:1-113) on one cell, arguments by
+ * pointer in the reference's order; pres, rho, vx, vy, vz are updated in place.
+ * pmean, undefined in the reference's active build, is (pm1+pm2)*0.5 as in
+ * hydro/ppm.F90:583. */
+int hp_integrate_cell(const hp_params* prm,
+                      const double* pm1, const double* pm2, const double* rhom1, const double* rhom2,
+                      const double* vxm1, const double* vxm2, const double* vym1, const double* vym2,
+                      const double* vzm1, const double* vzm2,
+                      double* pres, double* rho, double* vx, double* vy, double* vz,
+                      const double* etot, const double* eint, const double* ghalf);
+
+/* The dual-energy selection inlined at hydro/integrate.F90:182-217:
+ * given the updated total energy etotnew, density and velocities, the
+ * internal-energy-equation value eintnew_in and the old internal energy and
+ * pressure, returns the new pressure and internal energy. */
+int hp_dualenergy_cell(const hp_params* prm, double etotnew, double rhonew, double vx,
+                       double vy, double vz, double eintnew_in, double eint_old,
+                       double pres_old, double* pnew, double* eintnew);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYDROPAD_GPU_H */
