@@ -54,7 +54,8 @@ class HydropadError(RuntimeError):
 EXPORTS = ("hp_params_init", "hp_create", "hp_destroy", "hp_set_precision", "hp_upload",
            "hp_download", "hp_upload_device", "hp_download_device", "hp_set_params",
            "hp_ppmsolver", "hp_exchange_mesh", "hp_ppm_lxlylz", "hp_speed", "hp_evolve",
-           "hp_status", "hp_reset_counters", "hp_sync", "hp_integrate_cell", "hp_dualenergy_cell")
+           "hp_status", "hp_reset_counters", "hp_sync", "hp_integrate_cell", "hp_dualenergy_cell",
+           "hp_sweep_timing_enable", "hp_sweep_timing")
 
 _lib = None
 
@@ -88,6 +89,8 @@ def lib():
     L.hp_sync.argtypes = [_P]
     L.hp_integrate_cell.argtypes = [PP] + [_PD] * 18
     L.hp_dualenergy_cell.argtypes = [PP] + [_D] * 8 + [_PD, _PD]
+    L.hp_sweep_timing_enable.argtypes = [_P, _I]
+    L.hp_sweep_timing.argtypes = [_P, _PD, ctypes.POINTER(ctypes.c_int64)]
     _lib = L
     return L
 

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <new>
+#include <vector>
 
 #include "../../include/hydropad_gpu.h"
 #include "hp_core.h"
@@ -76,7 +77,22 @@ struct hp_ctx {
     int device;
     int seglen;
     int block;
+    bool timing;
+    std::vector<hipEvent_t> ev_pool;     // recycled events
+    std::vector<hipEvent_t> ev_used[3];  // start/end pairs per direction
 };
+
+static hipEvent_t take_event(hp_ctx* c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
 
 static bool is_cosmo(const hp_params& P) { return P.dat != 0.0 || P.at != 1.0 || P.rat != 1.0; }
 
@@ -106,7 +122,18 @@ static int sweep(hp_ctx* c, int dir, int ghost_mask, int do_speed)
     L.grav = c->has_grav;
     L.cosmo = is_cosmo(c->h_prm);
     hp::BoxView b = view_of(c);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (!e0 || !e1 || hipEventRecord(e0, c->stream) != hipSuccess) return HP_E_HIP;
+    }
     if (hp::launch_sweep(b, L, c->stream) != hipSuccess) return HP_E_HIP;
+    if (c->timing) {
+        if (hipEventRecord(e1, c->stream) != hipSuccess) return HP_E_HIP;
+        c->ev_used[dir - 1].push_back(e0);
+        c->ev_used[dir - 1].push_back(e1);
+    }
     c->cur = 1 - c->cur;
     return HP_OK;
 }
@@ -153,6 +180,7 @@ int hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out)
     c->ncell = (size_t)nx * ny * nz;
     c->seglen = 0;
     c->block = 256;
+    c->timing = false;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HP_E_HIP; }
     const size_t fbytes = c->ncell * sizeof(double);
     const size_t align = 256;
@@ -185,6 +213,9 @@ void hp_destroy(hp_ctx* c)
 {
     if (!c) return;
     if (c->stream) { hipStreamSynchronize(c->stream); hipStreamDestroy(c->stream); }
+    for (int d = 0; d < 3; ++d)
+        for (hipEvent_t e : c->ev_used[d]) hipEventDestroy(e);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->pool) hipFree(c->pool);
     delete c;
 }
@@ -318,6 +349,36 @@ int hp_evolve(hp_ctx* c, int first_step, int nsteps, double cour, double tstop, 
     HPCHK(hipStreamSynchronize(c->stream));
     *t = tt;
     *dt = c->h_prm.dt;
+    return HP_OK;
+}
+
+int hp_sweep_timing_enable(hp_ctx* c, int on)
+{
+    if (!c) return HP_E_ARG;
+    HPCHK(hipStreamSynchronize(c->stream));
+    for (int d = 0; d < 3; ++d) {
+        for (hipEvent_t e : c->ev_used[d]) c->ev_pool.push_back(e);
+        c->ev_used[d].clear();
+    }
+    c->timing = on != 0;
+    return HP_OK;
+}
+
+int hp_sweep_timing(hp_ctx* c, double ms[3], int64_t launches[3])
+{
+    if (!c || !ms || !launches) return HP_E_ARG;
+    HPCHK(hipStreamSynchronize(c->stream));
+    for (int d = 0; d < 3; ++d) {
+        double tot = 0.0;
+        const std::vector<hipEvent_t>& v = c->ev_used[d];
+        for (size_t i = 0; i + 1 < v.size(); i += 2) {
+            float m = 0.f;
+            HPCHK(hipEventElapsedTime(&m, v[i], v[i + 1]));
+            tot += m;
+        }
+        ms[d] = tot;
+        launches[d] = (int64_t)(v.size() / 2);
+    }
     return HP_OK;
 }
 

@@ -150,6 +150,16 @@ class Box:
         self.params.rdtath = dd.value
         return tt.value, dd.value
 
+    def sweep_timing_enable(self, on=True):
+        check("hp_sweep_timing_enable", self.L.hp_sweep_timing_enable(self._h, 1 if on else 0))
+
+    def sweep_timing(self):
+        """(ms[3], launches[3]) summed per sweep direction since enabling."""
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_int64 * 3)()
+        check("hp_sweep_timing", self.L.hp_sweep_timing(self._h, ms, n))
+        return list(ms), list(n)
+
     def counters(self):
         c = HpCounters()
         rc = self.L.hp_status(self._h, ctypes.byref(c))

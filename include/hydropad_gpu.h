@@ -121,6 +121,15 @@ int hp_speed(hp_ctx* ctx, double* velmax);
 int hp_evolve(hp_ctx* ctx, int first_step, int nsteps, double cour, double tstop,
               double* t, double* dt);
 
+/* Per-sweep kernel timing with HIP events on the context's stream; replaces
+ * the MPI_WTIME "Elapsed time" print around every ppmsolver call
+ * (hydro/ppmsolver.F90:72,464-465).  While enabled, every sweep launch is
+ * bracketed by two events; hp_sweep_timing synchronises and returns, per
+ * direction (index 0..2 = x,y,z), the summed kernel milliseconds and the
+ * number of launches since the last enable. */
+int hp_sweep_timing_enable(hp_ctx* ctx, int on);
+int hp_sweep_timing(hp_ctx* ctx, double ms[3], int64_t launches[3]);
+
 /* Accumulated counters since the last hp_reset_counters. */
 int hp_status(hp_ctx* ctx, hp_counters* out);
 int hp_reset_counters(hp_ctx* ctx);
