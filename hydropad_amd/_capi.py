@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libhydropad_amd.so")
 
 HP_OK, HP_E_ARG, HP_E_HIP, HP_E_NOMEM, HP_E_NEWTON, HP_E_STATE = 0, -1, -2, -3, -4, -5
-HP_FP64, HP_MIXED = 0, 1
+HP_FP64, HP_MIXED, HP_FP64_EXACT = 0, 1, 2
 _ERRNAMES = {HP_E_ARG: "HP_E_ARG", HP_E_HIP: "HP_E_HIP", HP_E_NOMEM: "HP_E_NOMEM",
              HP_E_NEWTON: "HP_E_NEWTON", HP_E_STATE: "HP_E_STATE"}
 

@@ -16,8 +16,10 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libhydropad_amd.so")
 ARCH = os.environ.get("HP_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["hp_kernels.hip", "hp_capi.hip"]
-HEADERS = ["hp_core.h", "hp_kernels.h", os.path.join("..", "..", "include", "hydropad_gpu.h")]
+# (source, FP contraction): the exact sweep keeps the reference's operation
+# order, so it is built without contraction; the throughput sweeps with it.
+SOURCES = [("hp_kernels.hip", "off"), ("hp_kernels_exact.hip", "off"), ("hp_kernels_fast.hip", "fast"),
+           ("hp_capi.hip", "off")]
 
 
 def hipcc():
@@ -27,22 +29,38 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def flags():
-    return ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-result"]
+def flags(contract):
+    return ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=" + contract,
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 
 
-def build(verbose=False):
-    """Compile the HIP library (always recompiles)."""
-    tmp = OUT + ".tmp"
-    cmd = [hipcc()] + flags() + ["-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+def _compile(args):
+    src, contract, obj, verbose = args
+    cmd = [hipcc()] + flags(contract) + ["-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
+    return src, r.returncode, r.stdout + r.stderr
+
+
+def build(verbose=False):
+    """Compile the HIP library (always recompiles; translation units in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = [(s, c, os.path.join(objdir, s.replace(".hip", ".o")), verbose) for s, c in SOURCES]
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        results = list(ex.map(_compile, jobs))
+    for src, rc, out in results:
+        if rc != 0:
+            sys.stderr.write(out)
+            raise RuntimeError(f"hipcc failed on {src}")
+    tmp = OUT + ".tmp"
+    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [j[2] for j in jobs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libhydropad_amd.so")
+        raise RuntimeError("linking libhydropad_amd.so failed")
     os.replace(tmp, OUT)
     return OUT
 

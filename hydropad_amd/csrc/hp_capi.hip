@@ -13,6 +13,8 @@
 #include "hp_kernels.h"
 
 static_assert(sizeof(hp_params) == sizeof(hp::Params), "hp_params and hp::Params must match");
+static_assert(HP_FP64 == hp::PREC_FP64 && HP_MIXED == hp::PREC_MIXED && HP_FP64_EXACT == hp::PREC_FP64_EXACT,
+              "precision codes");
 
 namespace {
 
@@ -172,7 +174,7 @@ int hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out)
     *out = nullptr;
     if (nbound != 3) return HP_E_ARG;   // the STENCIL path hard-wires a 6-cell stencil (ppmsolver.F90:82-84)
     if (nx < 2 * nbound + 3 || ny < 2 * nbound + 3 || nz < 2 * nbound + 3) return HP_E_ARG;
-    if (precision != HP_FP64 && precision != HP_MIXED) return HP_E_ARG;
+    if (precision != HP_FP64 && precision != HP_MIXED && precision != HP_FP64_EXACT) return HP_E_ARG;
     hp_ctx* c = new (std::nothrow) hp_ctx();
     if (!c) return HP_E_NOMEM;
     c->nx = nx; c->ny = ny; c->nz = nz; c->nb = nbound;
@@ -222,7 +224,7 @@ void hp_destroy(hp_ctx* c)
 
 int hp_set_precision(hp_ctx* c, int precision)
 {
-    if (!c || (precision != HP_FP64 && precision != HP_MIXED)) return HP_E_ARG;
+    if (!c || (precision != HP_FP64 && precision != HP_MIXED && precision != HP_FP64_EXACT)) return HP_E_ARG;
     c->precision = precision;
     return HP_OK;
 }

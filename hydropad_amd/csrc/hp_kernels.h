@@ -30,7 +30,7 @@ struct BoxView {
     int nx, ny, nz, nb;
 };
 
-enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
+enum Precision { PREC_FP64 = 0, PREC_MIXED = 1, PREC_FP64_EXACT = 2 };
 
 struct SweepLaunch {
     int dir;          // 1,2,3

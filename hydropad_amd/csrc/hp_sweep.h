@@ -1,0 +1,514 @@
+// hp_sweep.h -- the sweep kernels, templated on the math policy M (exact
+// reference-order arithmetic, or the throughput forms of hp_fast.h) and the
+// reconstruction precision R.  Included by hp_kernels_exact.hip (built with
+// FP contraction off) and hp_kernels_fast.hip (contraction on).
+//
+//   sweep_lane_x   x sweep.  One lane per cell of the contiguous axis; the
+//                  wave is a 64-cell window over the concatenated x-rows and
+//                  neighbours come from DPP wave shifts.  Lanes 3..60 own
+//                  their output, so a wave advances 58 cells.
+//   sweep_stream   y and z sweeps.  One thread walks a pencil segment along
+//                  the strided axis with a register sliding window; lanes of
+//                  a wave are consecutive x, so every load/store coalesces.
+// Both read the source buffer and write the other one (ping-pong), so pencils
+// can be split into segments without a read/write race, and they write the
+// periodic face-ghost images of what they store (the one-rank exchange_mesh,
+// hydropad/exchange_mesh.F90:47-123) in the same pass.
+#pragma once
+#include "hp_core.h"
+#include "hp_fast.h"
+#include "hp_kernels.h"
+
+namespace hp {
+
+// Reference-order arithmetic (hp_core.h).
+struct ExactMath {
+    template <class R> static __device__ __forceinline__ R slope(R am1, R a, R ap1) { return ajside(am1, a, ap1); }
+    template <class R> static __device__ __forceinline__ R iface(R am1, R a0, R dm1, R d0)
+    {
+        return iface_value(am1, a0, dm1, d0);
+    }
+    template <class R> static __device__ __forceinline__ bool steep(R a0, R a1, R thr) { return pair_steep(a0, a1, thr); }
+    template <class R>
+    static __device__ __forceinline__ void steepen_(R a1, R a2, R a3, R a4, R a5, R dml, R dmr, R dx, R& l, R& r)
+    {
+        steepen(a1, a2, a3, a4, a5, dml, dmr, dx, l, r);
+    }
+    template <class R> static __device__ __forceinline__ R addratio(R num, R den) { return num / den; }
+    template <class R> static __device__ __forceinline__ R sound(R gamma, R p, R rho) { return sqrt(gamma * p / rho); }
+    template <class R, bool G, bool C>
+    static __device__ __forceinline__ Face<R> right_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs, int& n)
+    {
+        return trace_right_face<R, G, C>(S, q, u, cs, n);
+    }
+    template <class R, bool G, bool C>
+    static __device__ __forceinline__ Face<R> left_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs, int& n)
+    {
+        return trace_left_face<R, G, C>(S, q, u, cs, n);
+    }
+    template <class R>
+    static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<R>& L, const Face<R>& Rt, int& nit)
+    {
+        return riemann_solve(Rs, L, Rt, nit);
+    }
+    static __device__ __forceinline__ void integrate(const IScal& S, const Flux& F1, const Flux& F2, double& p,
+                                                     double& rho, double& u, double& v, double& w, double etot,
+                                                     double g, bool grav)
+    {
+        integrate_cell(S, F1, F2, p, rho, u, v, w, etot, g, grav);
+    }
+    static __device__ __forceinline__ double vsound(double gamma, double p, double rho) { return sqrt(gamma * p / rho); }
+};
+
+// Throughput forms (hp_fast.h).
+struct FastMath {
+    template <class R> static __device__ __forceinline__ R slope(R am1, R a, R ap1) { return fm::ajside(am1, a, ap1); }
+    template <class R> static __device__ __forceinline__ R iface(R am1, R a0, R dm1, R d0)
+    {
+        return fm::iface_value(am1, a0, dm1, d0);
+    }
+    template <class R> static __device__ __forceinline__ bool steep(R a0, R a1, R thr) { return fm::pair_steep(a0, a1, thr); }
+    template <class R>
+    static __device__ __forceinline__ void steepen_(R a1, R a2, R a3, R a4, R a5, R dml, R dmr, R, R& l, R& r)
+    {
+        fm::steepen(a1, a2, a3, a4, a5, dml, dmr, l, r);
+    }
+    template <class R> static __device__ __forceinline__ R addratio(R num, R den) { return fm::div(num, den); }
+    template <class R> static __device__ __forceinline__ R sound(R gamma, R p, R rho)
+    {
+        return fm::fsqrt(gamma * p * fm::rcp(rho));
+    }
+    template <class R, bool G, bool C>
+    static __device__ __forceinline__ Face<R> right_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs, int& n)
+    {
+        return fm::trace_right_face<R, G, C>(S, q, u, cs, n);
+    }
+    template <class R, bool G, bool C>
+    static __device__ __forceinline__ Face<R> left_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs, int& n)
+    {
+        return fm::trace_left_face<R, G, C>(S, q, u, cs, n);
+    }
+    template <class R>
+    static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<R>& L, const Face<R>& Rt, int& nit)
+    {
+        return fm::riemann_solve(Rs, L, Rt, nit);
+    }
+    static __device__ __forceinline__ void integrate(const IScal& S, const Flux& F1, const Flux& F2, double& p,
+                                                     double& rho, double& u, double& v, double& w, double etot,
+                                                     double g, bool grav)
+    {
+        fm::integrate_cell(S, F1, F2, p, rho, u, v, w, etot, g, grav);
+    }
+    static __device__ __forceinline__ double vsound(double gamma, double p, double rho)
+    {
+        return sqrt(gamma * p * fm::rcp(rho));
+    }
+};
+
+// field index (0 p, 1 rho, 2 vx, 3 vy, 4 vz) of the normal velocity and the
+// transverse pair, hydro/ppmsolver.F90:105-107 (x), :227-229 (y), :355-357 (z)
+template <int DIR> struct Dirmap;
+template <> struct Dirmap<1> { static constexpr int U = 2, V = 3, W = 4; };
+template <> struct Dirmap<2> { static constexpr int U = 3, V = 2, W = 4; };
+template <> struct Dirmap<3> { static constexpr int U = 4, V = 3, W = 2; };
+
+// total(p,rho,vx,vy,vz) with the physical components in x,y,z order
+// (hydro/total.F90:15; ppmsolver.F90:149,277,405 pass vx3d,vy3d,vz3d).
+template <int DIR>
+__device__ __forceinline__ double total_phys(double rgamma1, double p, double r, double u, double v, double w)
+{
+    double vx, vy, vz;
+    if (DIR == 1) { vx = u; vy = v; vz = w; }
+    else if (DIR == 2) { vx = v; vy = u; vz = w; }
+    else { vx = w; vy = v; vz = u; }
+    return p * rgamma1 + 0.5 * r * (vx * vx + vy * vy + vz * vz);
+}
+
+// ---- DPP whole-wave shifts (gfx9 wave_shr:1 / wave_shl:1) -------------------
+constexpr int DPP_WAVE_SHL1 = 0x130;   // lane l receives lane l+1
+constexpr int DPP_WAVE_SHR1 = 0x138;   // lane l receives lane l-1
+
+template <int CTRL> __device__ __forceinline__ int dpp_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ double dpp(double x)
+{
+    int lo = dpp_i<CTRL>(__double2loint(x));
+    int hi = dpp_i<CTRL>(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL> __device__ __forceinline__ float dpp(float x)
+{
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+}
+template <class T> __device__ __forceinline__ T from_left(T x) { return dpp<DPP_WAVE_SHR1>(x); }
+template <class T> __device__ __forceinline__ T from_right(T x) { return dpp<DPP_WAVE_SHL1>(x); }
+
+template <class R> __device__ __forceinline__ Face<R> from_left(const Face<R>& f)
+{
+    Face<R> o;
+    o.p = from_left(f.p); o.r = from_left(f.r); o.u = from_left(f.u);
+    o.v = from_left(f.v); o.w = from_left(f.w);
+    return o;
+}
+__device__ __forceinline__ Flux from_right(const Flux& f)
+{
+    Flux o;
+    o.p = from_right(f.p); o.r = from_right(f.r); o.u = from_right(f.u);
+    o.v = from_right(f.v); o.w = from_right(f.w);
+    return o;
+}
+
+// ---- per-box scalars ----------------------------------------------------------
+template <class R> struct AllScalars {
+    SweepScalars<R> S;
+    RScal Rs;
+    IScal Is;
+    double rgamma1, gamma, mingradflat;
+};
+
+template <class R> __device__ __forceinline__ AllScalars<R> load_scalars(const Params* __restrict__ prm)
+{
+    const Params P = *prm;
+    AllScalars<R> A;
+    A.S.gamma = (R)P.gamma; A.S.dt = (R)P.dt; A.S.rat = (R)P.rat; A.S.dx = (R)P.dx;
+    A.S.rdx = (R)P.rdx; A.S.dat = (R)P.dat; A.S.at = (R)P.at;
+    A.S.flatvalue = (R)P.flatvalue; A.S.mingradflat = (R)P.mingradflat;
+    A.S.dtrat = (R)(P.dt * P.rat);
+    A.Rs.gamma = P.gamma; A.Rs.gamma1 = P.gamma1; A.Rs.rgamma = P.rgamma; A.Rs.rgamma1 = P.rgamma1;
+    A.Rs.m = P.m; A.Rs.rm = P.rm; A.Rs.gf = P.gf;
+    A.Rs.dadt = 1.0 - 0.5 * P.dat * P.dt * P.rat;     // riemannsolver.F90:36
+    A.Rs.dmax = P.dmax;
+    A.Is.rdtath = P.rdtath; A.Is.rdx = P.rdx; A.Is.gamma = P.gamma; A.Is.rgamma1 = P.rgamma1;
+    A.Is.eta1 = P.eta1; A.Is.eta2 = P.eta2;
+    A.rgamma1 = P.rgamma1; A.gamma = P.gamma; A.mingradflat = P.mingradflat;
+    return A;
+}
+
+// Parabolas of one cell and its two traced face states.
+//   a[]            the cell's values (p, rho, u, v, w[, g])
+//   ivl[], ivr[]   interface values before limiting at its left / right interface
+//   rm2..rp2       rho of cells c-2..c+2 (steepening), drm1/drp1 rho slopes of c-1, c+1
+//   pm1, pp1       p of cells c-1, c+1
+template <class M, class R, int NV, bool GRAV, bool COSMO>
+__device__ __forceinline__ void cell_faces(const SweepScalars<R>& S, const R* a, const R* ivl, const R* ivr,
+                                           R rm2, R rm1, R rp1, R rp2, R drm1, R drp1, R pm1, R pp1, bool flat,
+                                           Face<R>& fL, Face<R>& fR, int& nclamp)
+{
+    const R flatv = flat ? S.flatvalue : R(0);
+    const R addp = M::addratio(fabs(pp1 - pm1), tmin(pm1, pp1));              // order2interp.F90:152
+    const R addrho = M::addratio(R(0.5) * S.gamma * fabs(rp1 - rm1), tmin(rm1, rp1));   // :153
+    Parab<R> q[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        R ajl = ivl[v], ajr = ivr[v];
+        if (v == VR && addrho >= addp) M::steepen_(rm2, rm1, a[VR], rp1, rp2, drm1, drp1, S.dx, ajl, ajr);
+        q[v] = limit_parab(a[v], ajl, ajr, flatv);
+    }
+    const R cs = M::sound(S.gamma, a[VP], a[VR]);                             // ppmsolver.F90:109
+    fR = M::template right_face<R, GRAV, COSMO>(S, q, a[VU], cs, nclamp);
+    fL = M::template left_face<R, GRAV, COSMO>(S, q, a[VU], cs, nclamp);
+}
+
+// Store one updated cell and, for the axes in `mask`, its periodic face-ghost
+// image (ghost g <- cell g+N on the low side, ghost N+nb+g <- cell nb+g).
+__device__ __forceinline__ void store_cell(double* const* out, long q, const double* val, int ix, int iy, int iz,
+                                           int mask, int nx, int ny, int nz, int nb)
+{
+#pragma unroll
+    for (int f = 0; f < 5; ++f) out[f][q] = val[f];
+    if (mask) {
+        long shift[6];
+        int ns = 0;
+        if (mask & 1) {
+            const int N = nx - 2 * nb;
+            if (ix < 2 * nb) shift[ns++] = N;
+            if (ix >= N) shift[ns++] = -N;
+        }
+        if (mask & 2) {
+            const long N = ny - 2 * nb;
+            if (iy < 2 * nb) shift[ns++] = N * nx;
+            if (iy >= N) shift[ns++] = -N * nx;
+        }
+        if (mask & 4) {
+            const long N = nz - 2 * nb;
+            if (iz < 2 * nb) shift[ns++] = N * nx * (long)ny;
+            if (iz >= N) shift[ns++] = -N * nx * (long)ny;
+        }
+        for (int s = 0; s < ns; ++s) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) out[f][q + shift[s]] = val[f];
+        }
+    }
+}
+
+// Wave-level reduction of the per-lane counters, then one atomic per wave.
+__device__ __forceinline__ void flush_counters(DevStatus* st, int nitmax, int nonconv, int nclamp, double vmax,
+                                               bool want_speed)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        nitmax = max(nitmax, __shfl_xor(nitmax, off, 64));
+        nonconv += __shfl_xor(nonconv, off, 64);
+        nclamp += __shfl_xor(nclamp, off, 64);
+        if (want_speed) vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (nitmax > 0) atomicMax(&st->newton_max, nitmax);
+        if (nonconv) atomicAdd(&st->newton_nonconv, (unsigned long long)nonconv);
+        if (nclamp) atomicAdd(&st->clamps, (unsigned long long)nclamp);
+        if (want_speed) atomicMax(&st->velmax_bits, (unsigned long long)__double_as_longlong(vmax));
+    }
+}
+
+// Newton bookkeeping of one interface.
+__device__ __forceinline__ void count_newton(int nit, int& nitmax, int& nonconv)
+{
+    if (nit < 0) ++nonconv;
+    else nitmax = max(nitmax, nit);
+}
+
+// ============================================================================
+// x sweep: lane per cell, DPP neighbours.
+template <class M, class R, bool GRAV, bool COSMO>
+__global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, int do_speed)
+{
+    using D = Dirmap<1>;
+    constexpr int NV = GRAV ? 6 : 5;
+    const int nx = B.nx, ny = B.ny, nz = B.nz, nb = B.nb;
+    const int Nx = nx - 2 * nb, Ny = ny - 2 * nb;
+    const long rows = (long)Ny * (nz - 2 * nb);
+    const long total = rows * nx;                      // concatenated x-rows, ghosts included
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave * 58 - 3 >= total) return;                // whole wave past the end (uniform)
+    const AllScalars<R> A = load_scalars<R>(B.prm);
+
+    const long v = wave * 58 + lane - 3;
+    const bool in_range = v >= 0 && v < total;
+    const long vc = v < 0 ? 0 : (v >= total ? total - 1 : v);
+    const long row = vc / nx;
+    const int i = (int)(vc - row * nx);
+    const int j = (int)(row % Ny) + nb;
+    const int k = (int)(row / Ny) + nb;
+    const long q = i + (long)nx * (j + (long)ny * k);
+
+    double cd[NV];
+    cd[VP] = B.in[0][q]; cd[VR] = B.in[1][q];
+    cd[VU] = B.in[D::U][q]; cd[VV] = B.in[D::V][q]; cd[VW] = B.in[D::W][q];
+    if (GRAV) cd[NV - 1] = -B.gforce[3 * q + 0];
+
+    R a[NV], am1[NV], ap1[NV], d[NV], dm1[NV], iv[NV], ivr[NV];
+#pragma unroll
+    for (int s = 0; s < NV; ++s) {
+        a[s] = (R)cd[s];
+        am1[s] = from_left(a[s]);
+        ap1[s] = from_right(a[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < NV; ++s) d[s] = M::slope(am1[s], a[s], ap1[s]);   // slope of this cell
+#pragma unroll
+    for (int s = 0; s < NV; ++s) {
+        dm1[s] = from_left(d[s]);
+        iv[s] = M::iface(am1[s], a[s], dm1[s], d[s]);                // interface c-1/2
+    }
+#pragma unroll
+    for (int s = 0; s < NV; ++s) ivr[s] = from_right(iv[s]);          // interface c+1/2
+
+    // flattening: pair bit of (c-1,c), OR over pairs c-2..c+2 via the wave mask
+    bool bpair = false;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) bpair |= M::steep(am1[s], a[s], (R)A.mingradflat);
+    const uint64_t Mb = __ballot(bpair);
+    const uint64_t Mf = (Mb << 1) | Mb | (Mb >> 1) | (Mb >> 2);
+    const bool flat = (A.mingradflat <= 0.0) || ((Mf >> lane) & 1ull);
+
+    const R rm2 = from_left(am1[VR]);
+    const R rp2 = from_right(ap1[VR]);
+    const R drp1 = from_right(d[VR]);
+    int nclamp = 0, nitmax = 0, nonconv = 0;
+    Face<R> fL, fR;
+    cell_faces<M, R, NV, GRAV, COSMO>(A.S, a, iv, ivr, rm2, am1[VR], ap1[VR], rp2, dm1[VR], drp1, am1[VP],
+                                   ap1[VP], flat, fL, fR, nclamp);
+
+    // Riemann problem at interface c-1/2: right face of c-1 | left face of c
+    const Face<R> fRm1 = from_left(fR);
+    int nit;
+    Flux F = M::riemann(A.Rs, fRm1, fL, nit);
+    // high-density-peak floor (riemannsolver.F90:279-285), needs rho c-3..c
+    const uint64_t M1 = __ballot((double)a[VR] > A.Rs.dmax * (double)am1[VR]);
+    const uint64_t M2 = __ballot((double)am1[VR] > A.Rs.dmax * (double)a[VR]);
+    const uint64_t Mc = (M1 << 2) & M2;
+    if (Mc) {
+        const double r2 = (double)rm2, r1 = (double)from_left(rm2);
+        const double u3 = (double)am1[VU], u2 = (double)from_left(am1[VU]);
+        if ((Mc >> lane) & 1ull) peak_floor(A.Rs, F, r1, r2, (double)am1[VR], (double)a[VR], u2, u3);
+    }
+    const Flux F2 = from_right(F);                                    // interface c+1/2
+
+    const bool valid = in_range && lane >= 3 && lane <= 60 && i >= nb && i < nb + Nx;
+    if (valid) count_newton(nit, nitmax, nonconv);
+    double vmax = 0.0;
+    if (valid) {
+        double pres = cd[VP], rho = cd[VR], u = cd[VU], vv = cd[VV], w = cd[VW];
+        const double etot = total_phys<1>(A.rgamma1, pres, rho, u, vv, w);
+        M::integrate(A.Is, F, F2, pres, rho, u, vv, w, etot, GRAV ? cd[NV - 1] : 0.0, GRAV);
+        double val[5];
+        val[0] = pres; val[1] = rho; val[D::U] = u; val[D::V] = vv; val[D::W] = w;
+        store_cell(B.out, q, val, i, j, k, ghost_mask, nx, ny, nz, nb);
+        if (do_speed) {
+            const double vs = M::vsound(A.gamma, pres, rho);
+            vmax = tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs;
+        }
+    } else {
+        nclamp = 0;
+    }
+    flush_counters(B.st, nitmax, nonconv, nclamp, vmax, do_speed != 0);
+}
+
+// ============================================================================
+// y / z sweeps: one thread streams along a pencil segment.
+template <class M, class R, int DIR, bool GRAV, bool COSMO>
+__global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, int do_speed, int seglen, int nseg)
+{
+    using D = Dirmap<DIR>;
+    constexpr int NV = GRAV ? 6 : 5;
+    const int nx = B.nx, ny = B.ny, nz = B.nz, nb = B.nb;
+    const int Nx = nx - 2 * nb;
+    int Nax, Nbb;
+    long s_ax, s_b;
+    if (DIR == 2) { Nax = ny - 2 * nb; Nbb = nz - 2 * nb; s_ax = nx; s_b = (long)nx * ny; }
+    else { Nax = nz - 2 * nb; Nbb = ny - 2 * nb; s_ax = (long)nx * ny; s_b = nx; }
+    const long npen = (long)Nx * Nbb;
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = tid < npen * nseg;
+    const long tt = active ? tid : 0;
+    const int seg = (int)(tt / npen);
+    const long pen = tt - (long)seg * npen;
+    const int ia = (int)(pen % Nx), ib = (int)(pen / Nx);
+    const int s0 = seg * seglen;
+    const int s1 = min(s0 + seglen, Nax);
+    const long base = (nb + ia) + (long)(nb + ib) * s_b + (long)nb * s_ax;   // interior cell 0 of the pencil
+    const AllScalars<R> A = load_scalars<R>(B.prm);
+    const R thr = (R)A.mingradflat;
+    const bool flat_always = A.mingradflat <= 0.0;
+
+    double w0[NV], w1[NV], w2[NV], w3[NV];   // cells k-4 .. k-1
+    double r5 = 0.0;                         // rho of cell k-5
+    R d1[NV], d2[NV];                        // slopes of cells k-3, k-2
+    R iv2[NV];                               // interface value at k-2 (left of cell k-2)
+    unsigned bits = 0;                       // pair bits of k-3, k-2, k-1
+    Face<R> fRprev;                          // right face of cell k-3
+    Flux Fprev;                              // flux at interface k-3 (left of cell k-3)
+#pragma unroll
+    for (int s = 0; s < NV; ++s) { w0[s] = w1[s] = w2[s] = w3[s] = 0.0; d1[s] = d2[s] = iv2[s] = R(0); }
+    fRprev.p = fRprev.r = fRprev.u = fRprev.v = fRprev.w = R(0);
+    Fprev.p = Fprev.r = Fprev.u = Fprev.v = Fprev.w = 0.0;
+    int nclamp = 0, nitmax = 0, nonconv = 0;
+    double vmax = 0.0;
+
+    if (active) {
+        for (int kk = s0 - 3; kk <= s1 + 2; ++kk) {
+            const long q = base + (long)kk * s_ax;
+            double cur[NV];
+            cur[VP] = B.in[0][q]; cur[VR] = B.in[1][q];
+            cur[VU] = B.in[D::U][q]; cur[VV] = B.in[D::V][q]; cur[VW] = B.in[D::W][q];
+            if (GRAV) cur[NV - 1] = -B.gforce[3 * q + (DIR - 1)];
+
+            R d3[NV], iv1[NV];
+            bool bk = false;
+            if (kk >= s0 - 1) {
+#pragma unroll
+                for (int s = 0; s < NV; ++s) d3[s] = M::slope((R)w2[s], (R)w3[s], (R)cur[s]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < NV; ++s) d3[s] = R(0);
+            }
+            if (kk >= s0 - 2) {
+#pragma unroll
+                for (int s = 0; s < 5; ++s) bk |= M::steep((R)w3[s], (R)cur[s], thr);
+            }
+#pragma unroll
+            for (int s = 0; s < NV; ++s) iv1[s] = M::iface((R)w2[s], (R)w3[s], d2[s], d3[s]);
+
+            Face<R> fL = fRprev, fR = fRprev;
+            if (kk >= s0 + 1) {     // parabola and faces of cell k-2
+                const bool flat = flat_always || bits != 0u || bk;
+                R a[NV];
+#pragma unroll
+                for (int s = 0; s < NV; ++s) a[s] = (R)w2[s];
+                cell_faces<M, R, NV, GRAV, COSMO>(A.S, a, iv2, iv1, (R)w0[VR], (R)w1[VR], (R)w3[VR], (R)cur[VR],
+                                               d1[VR], d3[VR], (R)w1[VP], (R)w3[VP], flat, fL, fR, nclamp);
+            }
+            Flux F = Fprev;
+            if (kk >= s0 + 2) {     // Riemann problem at interface k-2
+                int nit;
+                F = M::riemann(A.Rs, fRprev, fL, nit);
+                peak_floor(A.Rs, F, r5, w0[VR], w1[VR], w2[VR], w0[VU], w1[VU]);
+                count_newton(nit, nitmax, nonconv);
+            }
+            if (kk >= s0 + 3) {     // update cell k-3
+                double pres = w1[VP], rho = w1[VR], u = w1[VU], vv = w1[VV], w = w1[VW];
+                const double etot = total_phys<DIR>(A.rgamma1, pres, rho, u, vv, w);
+                M::integrate(A.Is, Fprev, F, pres, rho, u, vv, w, etot, GRAV ? w1[NV - 1] : 0.0, GRAV);
+                double val[5];
+                val[0] = pres; val[1] = rho; val[D::U] = u; val[D::V] = vv; val[D::W] = w;
+                const int ic = nb + kk - 3;
+                int ix = nb + ia, iy, iz;
+                if (DIR == 2) { iy = ic; iz = nb + ib; }
+                else { iy = nb + ib; iz = ic; }
+                store_cell(B.out, q - 3 * s_ax, val, ix, iy, iz, ghost_mask, nx, ny, nz, nb);
+                if (do_speed) {
+                    const double vs = M::vsound(A.gamma, pres, rho);
+                    vmax = tmax(vmax, tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs);
+                }
+            }
+            // slide the window
+            r5 = w0[VR];
+#pragma unroll
+            for (int s = 0; s < NV; ++s) {
+                w0[s] = w1[s]; w1[s] = w2[s]; w2[s] = w3[s]; w3[s] = cur[s];
+                d1[s] = d2[s]; d2[s] = d3[s]; iv2[s] = iv1[s];
+            }
+            bits = (bits >> 1) | ((bk ? 1u : 0u) << 2);
+            fRprev = fR;
+            Fprev = F;
+        }
+    }
+    flush_counters(B.st, nitmax, nonconv, nclamp, vmax, do_speed != 0);
+}
+
+// host launcher of one sweep
+template <class M, class R, bool G, bool C>
+static hipError_t launch_sweep_t(const BoxView& b, const SweepLaunch& L, hipStream_t s)
+{
+    const int nb = b.nb;
+    const int Nx = b.nx - 2 * nb, Ny = b.ny - 2 * nb, Nz = b.nz - 2 * nb;
+    const int block = L.block > 0 ? L.block : 256;
+    if (L.dir == 1) {
+        const long total = (long)Ny * Nz * b.nx;
+        const long waves = (total + 57) / 58;
+        const long threads = waves * 64;
+        const long grid = (threads + block - 1) / block;
+        hipLaunchKernelGGL((sweep_lane_x<M, R, G, C>), dim3((unsigned)grid), dim3(block), 0, s, b, L.ghost_mask,
+                           L.do_speed);
+    } else {
+        const int Nax = (L.dir == 2) ? Ny : Nz;
+        const int Nbb = (L.dir == 2) ? Nz : Ny;
+        int seglen = L.seglen > 0 ? L.seglen : Nax;
+        if (seglen > Nax) seglen = Nax;
+        const int nseg = (Nax + seglen - 1) / seglen;
+        const long threads = (long)Nx * Nbb * nseg;
+        const long grid = (threads + block - 1) / block;
+        if (L.dir == 2)
+            hipLaunchKernelGGL((sweep_stream<M, R, 2, G, C>), dim3((unsigned)grid), dim3(block), 0, s, b,
+                               L.ghost_mask, L.do_speed, seglen, nseg);
+        else
+            hipLaunchKernelGGL((sweep_stream<M, R, 3, G, C>), dim3((unsigned)grid), dim3(block), 0, s, b,
+                               L.ghost_mask, L.do_speed, seglen, nseg);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hp

@@ -18,11 +18,11 @@ import ctypes
 import numpy as np
 
 from . import _capi
-from ._capi import HP_FP64, HP_MIXED, HpCounters, HpParams, check
+from ._capi import HP_FP64, HP_FP64_EXACT, HP_MIXED, HpCounters, HpParams, check
 
 FIELDS = ("p", "rho", "vx", "vy", "vz")
 NBOUND = 3
-PRECISIONS = {"fp64": HP_FP64, "mixed": HP_MIXED}
+PRECISIONS = {"fp64": HP_FP64, "mixed": HP_MIXED, "fp64_exact": HP_FP64_EXACT}
 
 
 def _ptr(a):

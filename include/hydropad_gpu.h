@@ -37,8 +37,10 @@ extern "C" {
 #define HP_E_STATE      -5   /* call not valid in the context's current state */
 
 /* ---- precision forms ---------------------------------------------------- */
-#define HP_FP64          0   /* everything in FP64 */
+#define HP_FP64          0   /* everything in FP64 (throughput arithmetic, same algorithm) */
 #define HP_MIXED         1   /* FP32 reconstruction/tracing/fluxes; FP64 Newton + dual energy + state */
+#define HP_FP64_EXACT    2   /* FP64 in the reference's own operation order, no FMA contraction:
+                                bit-faithful to the reference up to OCML-vs-glibc pow rounding */
 
 /*
  * The 19 module scalars the reference pushes to the device before every sweep
@@ -71,7 +73,7 @@ int hp_params_init(double gamma, int ngrid_max, double dt, hp_params* out);
 /* ---- context ------------------------------------------------------------ */
 /* Replaces the OpenACC `declare create` of the module fields (hydropad/matrix.F90:24)
  * and the per-sweep temporaries of hydro/ppmsolver.F90:55-64: device state is
- * allocated once here and stays resident. precision is HP_FP64 or HP_MIXED. */
+ * allocated once here and stays resident. precision is HP_FP64, HP_MIXED or HP_FP64_EXACT. */
 int  hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out);
 void hp_destroy(hp_ctx* ctx);
 int  hp_set_precision(hp_ctx* ctx, int precision);

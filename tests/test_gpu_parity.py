@@ -42,20 +42,38 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
 @pytest.mark.parametrize("name,make,N", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("d", [1, 2, 3])
-def test_sweep_vs_oracle(oracle, name, make, N, d):
+def test_sweep_vs_oracle(oracle, name, make, N, d, precision):
     st = make()
     P = oracle.params(ngrid=N)
     vm = oracle.speed(P, st)
     oracle.timedet(P, vm, 0.8, 1e30, 0.0)
     ref = copy_state(st)
     oracle.ppmsolver(P, ref, d)
-    out, c = gpu_sweep(st, P, d)
+    out, c = gpu_sweep(st, P, d, precision)
     assert_close(out, ref, what=(name, d))
-    same = np.mean([np.mean(out[k] == ref[k]) for k in FIELDS])
-    assert same > 0.9, same                 # FP64 path keeps the reference's op order
+    if precision == "fp64_exact":           # the reference's operation order: mostly bit-equal
+        same = np.mean([np.mean(out[k][NB:-NB, NB:-NB, NB:-NB] == ref[k][NB:-NB, NB:-NB, NB:-NB])
+                        for k in FIELDS])
+        assert same > 0.9, same
     assert c.newton_nonconv == 0 and c.newton_iters_max >= 1
+
+
+@pytest.mark.parametrize("name,make,N", CASES[:3], ids=[c[0] for c in CASES[:3]])
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_mixed_sweep_vs_oracle(oracle, name, make, N, d):
+    """Mixed form: FP32 reconstruction/tracing/Riemann sampling, FP64 Newton
+    and update.  One sweep stays within 2e-6 relative L1 of the FP64 oracle."""
+    st = make()
+    P = oracle.params(ngrid=N)
+    oracle.timedet(P, oracle.speed(P, st), 0.8, 1e30, 0.0)
+    ref = copy_state(st)
+    oracle.ppmsolver(P, ref, d)
+    out, c = gpu_sweep(st, P, d, "mixed")
+    for k in FIELDS:
+        assert rel_l1(out[k], ref[k]) <= 2e-6, (name, d, k, rel_l1(out[k], ref[k]))
 
 
 def test_sweep_noncubic_gravity_cosmo(oracle):
@@ -97,7 +115,8 @@ def test_sweeps_vs_reference_vectors(golden):
             for i, k in enumerate(FIELDS):
                 got = out[k][NB:-NB, NB:-NB, NB:-NB]
                 den = np.abs(want[i]).sum()
-                assert np.abs(got - want[i]).sum() / den <= TOL_L1, (name, d, k)
+                err = np.abs(got - want[i]).sum()
+                assert (err / den if den > 0 else err) <= TOL_L1, (name, d, k)
 
 
 def test_steps_vs_reference_vectors(golden):
@@ -116,21 +135,34 @@ def test_steps_vs_reference_vectors(golden):
     assert abs(dt - g["dts"][-1]) <= 1e-12 * g["dts"][-1]
 
 
-def test_sod64_100_steps(oracle):
+_SOD64 = {}
+
+
+def _sod64_oracle(oracle):
+    if "ref" not in _SOD64:
+        N, steps, cour = 64, 100, 0.8
+        st = ic.sod(N)
+        P = oracle.params(ngrid=N)
+        vm = oracle.speed(P, st)
+        dt0, t0 = oracle.timedet(P, vm, cour, 1e30, 0.0)
+        ref = copy_state(st)
+        oracle.set_threads(16)
+        r, t_ref = oracle.evolve(P, ref, 1, steps, cour, 1e30, t0)
+        oracle.set_threads(1)
+        assert r >= 0
+        _SOD64.update(ref=ref, t=t_ref, dt=P.dt, vm=vm, dt0=dt0, st=st)
+    return _SOD64
+
+
+@pytest.mark.parametrize("precision", ["fp64_exact", "fp64"])
+def test_sod64_100_steps(oracle, precision):
     """The 64^3 shock-tube workload: 100 split-sweep steps at CFL 0.8 (nrot
     schedule, on-device speed + timedet) against the oracle, rel L1 <= 1e-12
     per field."""
     N, steps, cour = 64, 100, 0.8
-    st = ic.sod(N)
-    P = oracle.params(ngrid=N)
-    vm = oracle.speed(P, st)
-    dt0, t0 = oracle.timedet(P, vm, cour, 1e30, 0.0)
-    ref = copy_state(st)
-    oracle.set_threads(16)
-    r, t_ref = oracle.evolve(P, ref, 1, steps, cour, 1e30, t0)
-    oracle.set_threads(1)
-    assert r >= 0
-    with hp.Box(N) as box:
+    o = _sod64_oracle(oracle)
+    st, ref, t_ref, vm, dt0 = o["st"], o["ref"], o["t"], o["vm"], o["dt0"]
+    with hp.Box(N, precision) as box:
         box.upload(copy_state(st))
         vm2 = box.speed()
         assert vm2 == vm
@@ -139,11 +171,24 @@ def test_sod64_100_steps(oracle):
         t, dt = box.evolve(1, steps, cour, t, dt)
         out = box.download()
         c = box.counters()
-    assert_close(out, ref, what="sod64")
-    assert abs(t - t_ref) <= 1e-13 * t_ref and abs(dt - P.dt) <= 1e-12 * P.dt
+    assert_close(out, ref, what=("sod64", precision))
+    assert abs(t - t_ref) <= 1e-12 * t_ref and abs(dt - o["dt"]) <= 1e-11 * o["dt"]
     assert c.newton_nonconv == 0
     # the shock tube creates no transverse motion, exactly
     assert np.all(out["vy"] == 0.0) and np.all(out["vz"] == 0.0)
+
+
+def test_sod64_100_steps_mixed(oracle):
+    """Mixed form on the 64^3 workload: L1(rho) <= 1e-5 against the FP64 oracle."""
+    N, steps, cour = 64, 100, 0.8
+    o = _sod64_oracle(oracle)
+    with hp.Box(N, "mixed") as box:
+        box.upload(copy_state(o["st"]))
+        dt, t = box.timedet(box.speed(), cour)
+        box.evolve(1, steps, cour, t, dt)
+        out = box.download()
+    for k in ("p", "rho"):
+        assert rel_l1(out[k], o["ref"][k]) <= 1e-5, (k, rel_l1(out[k], o["ref"][k]))
 
 
 def test_sod64_matches_exact_solution():
