@@ -44,7 +44,22 @@ __device__ __forceinline__ double div(double a, double b)
 }
 __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
-__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+// sqrt for x >= 0 without the range scaling of the correctly-rounded
+// expansion: rsq seed and two Goldschmidt corrections (within 1 ulp for
+// normal x; 0 for x == 0).
+__device__ __forceinline__ double fsqrt(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x > 0.0 ? g : 0.0;
+}
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 constexpr double LN2_HI = 6.93147180369123816490e-01;
@@ -272,7 +287,7 @@ __device__ __forceinline__ Face<R> trace_left_face(const SweepScalars<R>& S, con
 __device__ __forceinline__ bool pcheck_guess(const RScal& Rs, double pleft, double pright, double rholeft,
                                              double rhoright, double vxleft, double vxright, double& pstar)
 {
-    const double kappa = sqrt(Rs.gamma * (pleft + pright) * (rholeft + rhoright) * 0.25);
+    const double kappa = fsqrt(Rs.gamma * (pleft + pright) * (rholeft + rhoright) * 0.25);
     const double pcheck = 0.5 * (pleft + pright) + 0.5 * kappa * (vxleft + vxright);
     if (pcheck > 0.0) {
         pstar = pcheck;
@@ -287,8 +302,8 @@ __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vs
                                       double rholeft, double rhoright, double vxleft, double vxright)
 {
     const double gamma = Rs.gamma, m = Rs.m, gf = Rs.gf;
-    const double cleft = sqrt(gamma * pleft * rholeft);
-    const double cright = sqrt(gamma * pright * rhoright);
+    const double cleft = fsqrt(gamma * pleft * rholeft);
+    const double cright = fsqrt(gamma * pright * rhoright);
     const double rpl = rcp(pleft), rpr = rcp(pright);
     const double rcl = rcp(cleft), rcr = rcp(cright);
     const double cl2 = cleft * cleft, cr2 = cright * cright;
@@ -299,19 +314,19 @@ __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vs
         double a, b, vpl, vpr;
         const double xl = pstar * rpl, xr = pstar * rpr;
         if (pstar >= pleft) {
-            a = cleft * sqrt(gf * xl + m);
+            a = cleft * fsqrt(gf * xl + m);
             vpl = -(a * a + cl2) * 0.5 * rcp(a * a * a);
         } else {
             const double w = pow_pos(xl, m);
-            a = (w == 1.0) ? cleft * sqrt(gf * xl + m) : m * cleft * div(1.0 - xl, 1.0 - w);
+            a = (w == 1.0) ? cleft * fsqrt(gf * xl + m) : m * cleft * div(1.0 - xl, 1.0 - w);
             vpl = -rcl * div(w, xl);
         }
         if (pstar >= pright) {
-            b = cright * sqrt(gf * xr + m);
+            b = cright * fsqrt(gf * xr + m);
             vpr = (b * b + cr2) * 0.5 * rcp(b * b * b);
         } else {
             const double w = pow_pos(xr, m);
-            b = (w == 1.0) ? cright * sqrt(gf * xr + m) : m * cright * div(1.0 - xr, 1.0 - w);
+            b = (w == 1.0) ? cright * fsqrt(gf * xr + m) : m * cright * div(1.0 - xr, 1.0 - w);
             vpr = rcr * div(w, xr);
         }
         const double ra = rcp(a), rb = rcp(b);

@@ -101,7 +101,7 @@ struct FastMath {
     }
     static __device__ __forceinline__ double vsound(double gamma, double p, double rho)
     {
-        return sqrt(gamma * p * fm::rcp(rho));
+        return fm::fsqrt(gamma * p * fm::rcp(rho));
     }
 };
 
@@ -272,97 +272,123 @@ __device__ __forceinline__ void count_newton(int nit, int& nitmax, int& nonconv)
 // ============================================================================
 // x sweep: lane per cell, DPP neighbours.
 template <class M, class R, bool GRAV, bool COSMO>
-__global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, int do_speed)
+__global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, int do_speed, int chunks_per_wave)
 {
     using D = Dirmap<1>;
     constexpr int NV = GRAV ? 6 : 5;
-    const int nx = B.nx, ny = B.ny, nz = B.nz, nb = B.nb;
-    const int Nx = nx - 2 * nb, Ny = ny - 2 * nb;
-    const long rows = (long)Ny * (nz - 2 * nb);
-    const long total = rows * nx;                      // concatenated x-rows, ghosts included
+    const int nx = B.nx, ny = B.ny, nb = B.nb;
+    const int Nx = nx - 2 * nb;
+    const unsigned Ny = (unsigned)(ny - 2 * nb);
+    const unsigned total = Ny * (unsigned)(B.nz - 2 * nb) * (unsigned)nx;   // concatenated x-rows
+    const unsigned nchunks = (total + 57u) / 58u;
     const int lane = threadIdx.x & 63;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (wave * 58 - 3 >= total) return;                // whole wave past the end (uniform)
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned c0 = wave * (unsigned)chunks_per_wave;
+    if (c0 >= nchunks) return;                         // whole wave past the end (uniform)
+    const unsigned c1 = min(c0 + (unsigned)chunks_per_wave, nchunks);
     const AllScalars<R> A = load_scalars<R>(B.prm);
 
-    const long v = wave * 58 + lane - 3;
-    const bool in_range = v >= 0 && v < total;
-    const long vc = v < 0 ? 0 : (v >= total ? total - 1 : v);
-    const long row = vc / nx;
-    const int i = (int)(vc - row * nx);
-    const int j = (int)(row % Ny) + nb;
-    const int k = (int)(row / Ny) + nb;
-    const long q = i + (long)nx * (j + (long)ny * k);
+    // cell of this lane in chunk c: virtual position 58c + lane - 3
+    auto locate = [&](unsigned c, long& q, int& i, int& j, int& k, bool& in_range) {
+        const int v = (int)(58u * c) + lane - 3;
+        in_range = v >= 0 && (unsigned)v < total;
+        const unsigned vc = v < 0 ? 0u : ((unsigned)v >= total ? total - 1u : (unsigned)v);
+        const unsigned row = vc / (unsigned)nx;
+        i = (int)(vc - row * (unsigned)nx);
+        const unsigned kz = row / Ny;
+        j = (int)(row - kz * Ny) + nb;
+        k = (int)kz + nb;
+        q = i + (long)nx * (j + (long)ny * k);
+    };
+    auto load = [&](long q, double* cd) {
+        cd[VP] = B.in[0][q]; cd[VR] = B.in[1][q];
+        cd[VU] = B.in[D::U][q]; cd[VV] = B.in[D::V][q]; cd[VW] = B.in[D::W][q];
+        if (GRAV) cd[NV - 1] = -B.gforce[3 * q + 0];
+    };
 
-    double cd[NV];
-    cd[VP] = B.in[0][q]; cd[VR] = B.in[1][q];
-    cd[VU] = B.in[D::U][q]; cd[VV] = B.in[D::V][q]; cd[VW] = B.in[D::W][q];
-    if (GRAV) cd[NV - 1] = -B.gforce[3 * q + 0];
-
-    R a[NV], am1[NV], ap1[NV], d[NV], dm1[NV], iv[NV], ivr[NV];
-#pragma unroll
-    for (int s = 0; s < NV; ++s) {
-        a[s] = (R)cd[s];
-        am1[s] = from_left(a[s]);
-        ap1[s] = from_right(a[s]);
-    }
-#pragma unroll
-    for (int s = 0; s < NV; ++s) d[s] = M::slope(am1[s], a[s], ap1[s]);   // slope of this cell
-#pragma unroll
-    for (int s = 0; s < NV; ++s) {
-        dm1[s] = from_left(d[s]);
-        iv[s] = M::iface(am1[s], a[s], dm1[s], d[s]);                // interface c-1/2
-    }
-#pragma unroll
-    for (int s = 0; s < NV; ++s) ivr[s] = from_right(iv[s]);          // interface c+1/2
-
-    // flattening: pair bit of (c-1,c), OR over pairs c-2..c+2 via the wave mask
-    bool bpair = false;
-#pragma unroll
-    for (int s = 0; s < 5; ++s) bpair |= M::steep(am1[s], a[s], (R)A.mingradflat);
-    const uint64_t Mb = __ballot(bpair);
-    const uint64_t Mf = (Mb << 1) | Mb | (Mb >> 1) | (Mb >> 2);
-    const bool flat = (A.mingradflat <= 0.0) || ((Mf >> lane) & 1ull);
-
-    const R rm2 = from_left(am1[VR]);
-    const R rp2 = from_right(ap1[VR]);
-    const R drp1 = from_right(d[VR]);
     int nclamp = 0, nitmax = 0, nonconv = 0;
-    Face<R> fL, fR;
-    cell_faces<M, R, NV, GRAV, COSMO>(A.S, a, iv, ivr, rm2, am1[VR], ap1[VR], rp2, dm1[VR], drp1, am1[VP],
-                                   ap1[VP], flat, fL, fR, nclamp);
-
-    // Riemann problem at interface c-1/2: right face of c-1 | left face of c
-    const Face<R> fRm1 = from_left(fR);
-    int nit;
-    Flux F = M::riemann(A.Rs, fRm1, fL, nit);
-    // high-density-peak floor (riemannsolver.F90:279-285), needs rho c-3..c
-    const uint64_t M1 = __ballot((double)a[VR] > A.Rs.dmax * (double)am1[VR]);
-    const uint64_t M2 = __ballot((double)am1[VR] > A.Rs.dmax * (double)a[VR]);
-    const uint64_t Mc = (M1 << 2) & M2;
-    if (Mc) {
-        const double r2 = (double)rm2, r1 = (double)from_left(rm2);
-        const double u3 = (double)am1[VU], u2 = (double)from_left(am1[VU]);
-        if ((Mc >> lane) & 1ull) peak_floor(A.Rs, F, r1, r2, (double)am1[VR], (double)a[VR], u2, u3);
-    }
-    const Flux F2 = from_right(F);                                    // interface c+1/2
-
-    const bool valid = in_range && lane >= 3 && lane <= 60 && i >= nb && i < nb + Nx;
-    if (valid) count_newton(nit, nitmax, nonconv);
     double vmax = 0.0;
-    if (valid) {
-        double pres = cd[VP], rho = cd[VR], u = cd[VU], vv = cd[VV], w = cd[VW];
-        const double etot = total_phys<1>(A.rgamma1, pres, rho, u, vv, w);
-        M::integrate(A.Is, F, F2, pres, rho, u, vv, w, etot, GRAV ? cd[NV - 1] : 0.0, GRAV);
-        double val[5];
-        val[0] = pres; val[1] = rho; val[D::U] = u; val[D::V] = vv; val[D::W] = w;
-        store_cell(B.out, q, val, i, j, k, ghost_mask, nx, ny, nz, nb);
-        if (do_speed) {
-            const double vs = M::vsound(A.gamma, pres, rho);
-            vmax = tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs;
+    long qn;
+    int in_, jn, kn;
+    bool rn;
+    double nxt[NV];
+    locate(c0, qn, in_, jn, kn, rn);
+    load(qn, nxt);
+    for (unsigned c = c0; c < c1; ++c) {
+        const long q = qn;
+        const int i = in_, j = jn, k = kn;
+        const bool in_range = rn;
+        double cd[NV];
+#pragma unroll
+        for (int s = 0; s < NV; ++s) cd[s] = nxt[s];
+        if (c + 1 < c1) {                              // prefetch the next chunk
+            locate(c + 1, qn, in_, jn, kn, rn);
+            load(qn, nxt);
         }
-    } else {
-        nclamp = 0;
+
+        R a[NV], am1[NV], ap1[NV], d[NV], dm1[NV], iv[NV], ivr[NV];
+#pragma unroll
+        for (int s = 0; s < NV; ++s) {
+            a[s] = (R)cd[s];
+            am1[s] = from_left(a[s]);
+            ap1[s] = from_right(a[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < NV; ++s) d[s] = M::slope(am1[s], a[s], ap1[s]);   // slope of this cell
+#pragma unroll
+        for (int s = 0; s < NV; ++s) {
+            dm1[s] = from_left(d[s]);
+            iv[s] = M::iface(am1[s], a[s], dm1[s], d[s]);                // interface c-1/2
+        }
+#pragma unroll
+        for (int s = 0; s < NV; ++s) ivr[s] = from_right(iv[s]);          // interface c+1/2
+
+        // flattening: pair bit of (c-1,c), OR over pairs c-2..c+2 via the wave mask
+        bool bpair = false;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) bpair |= M::steep(am1[s], a[s], (R)A.mingradflat);
+        const uint64_t Mb = __ballot(bpair);
+        const uint64_t Mf = (Mb << 1) | Mb | (Mb >> 1) | (Mb >> 2);
+        const bool flat = (A.mingradflat <= 0.0) || ((Mf >> lane) & 1ull);
+
+        const R rm2 = from_left(am1[VR]);
+        const R rp2 = from_right(ap1[VR]);
+        const R drp1 = from_right(d[VR]);
+        int ncl = 0;
+        Face<R> fL, fR;
+        cell_faces<M, R, NV, GRAV, COSMO>(A.S, a, iv, ivr, rm2, am1[VR], ap1[VR], rp2, dm1[VR], drp1, am1[VP],
+                                          ap1[VP], flat, fL, fR, ncl);
+
+        // Riemann problem at interface c-1/2: right face of c-1 | left face of c
+        const Face<R> fRm1 = from_left(fR);
+        int nit;
+        Flux F = M::riemann(A.Rs, fRm1, fL, nit);
+        // high-density-peak floor (riemannsolver.F90:279-285), needs rho c-3..c
+        const uint64_t M1 = __ballot((double)a[VR] > A.Rs.dmax * (double)am1[VR]);
+        const uint64_t M2 = __ballot((double)am1[VR] > A.Rs.dmax * (double)a[VR]);
+        const uint64_t Mc = (M1 << 2) & M2;
+        if (Mc) {
+            const double r2 = (double)rm2, r1 = (double)from_left(rm2);
+            const double u3 = (double)am1[VU], u2 = (double)from_left(am1[VU]);
+            if ((Mc >> lane) & 1ull) peak_floor(A.Rs, F, r1, r2, (double)am1[VR], (double)a[VR], u2, u3);
+        }
+        const Flux F2 = from_right(F);                                    // interface c+1/2
+
+        const bool valid = in_range && lane >= 3 && lane <= 60 && i >= nb && i < nb + Nx;
+        if (valid) {
+            count_newton(nit, nitmax, nonconv);
+            nclamp += ncl;
+            double pres = cd[VP], rho = cd[VR], u = cd[VU], vv = cd[VV], w = cd[VW];
+            const double etot = total_phys<1>(A.rgamma1, pres, rho, u, vv, w);
+            M::integrate(A.Is, F, F2, pres, rho, u, vv, w, etot, GRAV ? cd[NV - 1] : 0.0, GRAV);
+            double val[5];
+            val[0] = pres; val[1] = rho; val[D::U] = u; val[D::V] = vv; val[D::W] = w;
+            store_cell(B.out, q, val, i, j, k, ghost_mask, nx, ny, B.nz, nb);
+            if (do_speed) {
+                const double vs = M::vsound(A.gamma, pres, rho);
+                vmax = tmax(vmax, tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs);
+            }
+        }
     }
     flush_counters(B.st, nitmax, nonconv, nclamp, vmax, do_speed != 0);
 }
@@ -409,12 +435,24 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, i
     double vmax = 0.0;
 
     if (active) {
+        double nxt[NV];
+        {
+            const long q0 = base + (long)(s0 - 3) * s_ax;
+            nxt[VP] = B.in[0][q0]; nxt[VR] = B.in[1][q0];
+            nxt[VU] = B.in[D::U][q0]; nxt[VV] = B.in[D::V][q0]; nxt[VW] = B.in[D::W][q0];
+            if (GRAV) nxt[NV - 1] = -B.gforce[3 * q0 + (DIR - 1)];
+        }
         for (int kk = s0 - 3; kk <= s1 + 2; ++kk) {
             const long q = base + (long)kk * s_ax;
             double cur[NV];
-            cur[VP] = B.in[0][q]; cur[VR] = B.in[1][q];
-            cur[VU] = B.in[D::U][q]; cur[VV] = B.in[D::V][q]; cur[VW] = B.in[D::W][q];
-            if (GRAV) cur[NV - 1] = -B.gforce[3 * q + (DIR - 1)];
+#pragma unroll
+            for (int s = 0; s < NV; ++s) cur[s] = nxt[s];
+            if (kk + 1 <= s1 + 2) {                    // prefetch the next cell of the pencil
+                const long qn = q + s_ax;
+                nxt[VP] = B.in[0][qn]; nxt[VR] = B.in[1][qn];
+                nxt[VU] = B.in[D::U][qn]; nxt[VV] = B.in[D::V][qn]; nxt[VW] = B.in[D::W][qn];
+                if (GRAV) nxt[NV - 1] = -B.gforce[3 * qn + (DIR - 1)];
+            }
 
             R d3[NV], iv1[NV];
             bool bk = false;
@@ -488,11 +526,17 @@ static hipError_t launch_sweep_t(const BoxView& b, const SweepLaunch& L, hipStre
     const int block = L.block > 0 ? L.block : 256;
     if (L.dir == 1) {
         const long total = (long)Ny * Nz * b.nx;
-        const long waves = (total + 57) / 58;
+        const long nchunks = (total + 57) / 58;
+        // about 16 resident rounds of waves over 256 CUs x 4 SIMDs; each wave
+        // walks its run of chunks with a one-chunk register prefetch
+        long cpw = nchunks / 16384;
+        if (L.seglen > 0) cpw = L.seglen;
+        if (cpw < 1) cpw = 1;
+        const long waves = (nchunks + cpw - 1) / cpw;
         const long threads = waves * 64;
         const long grid = (threads + block - 1) / block;
         hipLaunchKernelGGL((sweep_lane_x<M, R, G, C>), dim3((unsigned)grid), dim3(block), 0, s, b, L.ghost_mask,
-                           L.do_speed);
+                           L.do_speed, (int)cpw);
     } else {
         const int Nax = (L.dir == 2) ? Ny : Nz;
         const int Nbb = (L.dir == 2) ? Nz : Ny;
