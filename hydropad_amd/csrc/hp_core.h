@@ -264,6 +264,7 @@ __device__ __forceinline__ Face<R> trace_left_face(const SweepScalars<R>& S, con
 
 struct RScal {
     double gamma, gamma1, rgamma, rgamma1, m, rm, gf, dadt, dmax;
+    int m5;   // m == 1/5 (gamma = 5/3): powers of m, 1/gamma, 1/m are roots/products
 };
 
 __device__ __forceinline__ double rarwave(double f1, double f2, double vt, double vh)

@@ -134,151 +134,229 @@ __device__ __forceinline__ float pow_pos(float x, float y)
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
 }
 
-// ---- reconstruction ------------------------------------------------------------
-template <class R> __device__ __forceinline__ R ajside(R am1, R a, R ap1)
+// x^(1/5) for x > 0: FP32 seed, then two Newton steps on y^5 = x,
+// y <- (4y + x/y^4)/5 (each step squares the relative error).
+__device__ __forceinline__ double root5(double x)
 {
-    if ((ap1 - a) * (a - am1) > R(0)) {
-        R da = R(0.5) * (ap1 - am1);
-        R mn = tmin(tmin(fabs(da), R(2) * fabs(a - am1)), R(2) * fabs(ap1 - a));
+    double y = (double)__builtin_amdgcn_exp2f(0.2f * __builtin_amdgcn_logf((float)x));
+    double y2 = y * y;
+    y = fma(0.8, y, 0.2 * x * rcp(y2 * y2));
+    y2 = y * y;
+    y = fma(0.8, y, 0.2 * x * rcp(y2 * y2));
+    return y;
+}
+
+// The three powers the Riemann solver takes: x^m, x^(1/gamma), x^(1/m).  For
+// gamma = 5/3 (m = 1/5) they are x^(1/5), (x^(1/5))^3 and x^5.
+__device__ __forceinline__ double pow_m(const RScal& R, double x) { return R.m5 ? root5(x) : pow_pos(x, R.m); }
+__device__ __forceinline__ double pow_rgamma(const RScal& R, double x)
+{
+    if (R.m5) {
+        const double y = root5(x);
+        return y * y * y;
+    }
+    return pow_pos(x, R.rgamma);
+}
+__device__ __forceinline__ double pow_rm(const RScal& R, double x)
+{
+    if (R.m5) {
+        const double x2 = x * x;
+        return x2 * x2 * x;
+    }
+    return pow_pos(x, R.rm);
+}
+
+// ---- reconstruction and tracing, delta form ---------------------------------------
+// Cell values stay FP64.  R (float in the mixed form, double in the FP64 form)
+// carries only differences and offsets from a cell's own value: one-sided
+// differences, MC slopes, the parabola's edge offsets (ajl - a, ajr - a) and
+// the characteristic corrections.  The parabola algebra is translation
+// invariant (a6 = 6(a - (ajl+ajr)/2) = -3(l+r) in offsets; the CW84 limits
+// become l = -2r / r = -2l; flattening scales the offsets by 1-flat), so FP32
+// rounding acts on the small offsets, not on the O(1) values.  Every switch
+// whose flip would be a jump -- flattening, the steepening switch, the
+// characteristic signs -- is decided in FP64 from the FP64 values.
+
+// MC slope (ppm_functs.F90:167-181) from the cell's one-sided differences.
+template <class R> __device__ __forceinline__ R slope_d(double am1, double a, double ap1)
+{
+    const R dm = (R)(a - am1), dp = (R)(ap1 - a);
+    if (dp * dm > R(0)) {
+        const R da = R(0.5) * (dm + dp);
+        const R mn = tmin(tmin(fabs(da), R(2) * fabs(dm)), R(2) * fabs(dp));
         return copysign(mn, da);
     }
     return R(0);
 }
 
-template <class R> __device__ __forceinline__ R iface_value(R am1, R a0, R dm1, R d0)
+// interface value between (c-1 | c) before limiting, FP64 (ppm_functs.F90:33-34)
+template <class R> __device__ __forceinline__ double iface_d(double am1, double a0, R dm1, R d0)
 {
-    return am1 + R(0.5) * (a0 - am1) + (dm1 - d0) * R(K_SIXTH);
+    return am1 + 0.5 * (a0 - am1) + (double)((dm1 - d0) * R(K_SIXTH));
 }
 
-template <class R> __device__ __forceinline__ bool pair_steep(R a0, R a1, R thr)
+// one pair of the flattening test, FP64 (order2interp.F90:65-113)
+__device__ __forceinline__ bool pair_steep_d(double a0, double a1, double thr)
 {
-    return (a1 * a0 != R(0)) && (fabs(a1 - a0) >= thr * fabs(tmin(a1, a0)));
+    return (a1 * a0 != 0.0) && (fabs(a1 - a0) >= thr * fabs(tmin(a1, a0)));
 }
 
-// contact steepening: deltam2, deltap2 carry C6/dx^2, which cancels in eta
+// contact steepening on offsets (dda.F90:182-216, ppm_functs.F90:35-40);
+// a1..a5 = rho(c-2..c+2), dmalf/dmarg the slopes of c-1 / c+1
 template <class R>
-__device__ __forceinline__ void steepen(R a1, R a2, R a3, R a4, R a5, R dmalf, R dmarg, R& ajl, R& ajr)
+__device__ __forceinline__ void steepen_d(double a1, double a2, double a3, double a4, double a5, R dmalf, R dmarg,
+                                          R& l, R& r)
 {
-    R dm = a3 - R(2) * a2 + a1;
-    R dp = a5 - R(2) * a4 + a3;
-    R agrad = fabs(a4 - a2) - R(K_0010) * tmin(fabs(a4), fabs(a2));
+    const double dm = (a3 - a2) - (a2 - a1);          // second differences; C6/dx^2 cancels in eta
+    const double dp = (a5 - a4) - (a4 - a3);
+    const double agrad = fabs(a4 - a2) - K_0010 * tmin(fabs(a4), fabs(a2));
     R eta = R(0);
-    if (-dm * dp > R(0) && agrad > R(0)) {
-        R e = -R(K_SIXTH) * div(dp - dm, a4 - a2);
+    if (-dm * dp > 0.0 && agrad > 0.0) {
+        const R e = -R(K_SIXTH) * div((R)(dp - dm), (R)(a4 - a2));
         eta = tmax(R(0), tmin(R(20) * (e - R(K_0050)), R(1)));
     }
-    ajl = (R(1) - eta) * ajl + eta * (a2 + R(0.5) * dmalf);
-    ajr = (R(1) - eta) * ajr + eta * (a4 - R(0.5) * dmarg);
+    l = (R(1) - eta) * l + eta * ((R)(a2 - a3) + R(0.5) * dmalf);
+    r = (R(1) - eta) * r + eta * ((R)(a4 - a3) - R(0.5) * dmarg);
 }
 
-// ---- characteristic tracing --------------------------------------------------------
+// CW84 monotonicity + flattening on edge offsets (ppm_functs.F90:41-54)
+template <class R> __device__ __forceinline__ Parab<R> limit_parab_d(R l, R r, R flat)
+{
+    R d = r - l;
+    R a6 = R(-3) * (l + r);
+    if (-(r * l) <= R(0)) {
+        l = R(0);
+        r = R(0);
+    } else if (d * a6 > d * d) {
+        l = R(-2) * r;
+    } else if (d * a6 < (-(d * d))) {
+        r = R(-2) * l;
+    }
+    l = (R(1) - flat) * l;
+    r = (R(1) - flat) * r;
+    Parab<R> q;
+    q.l = l;
+    q.r = r;
+    q.d = r - l;
+    q.a6 = R(-3) * (l + r);
+    return q;
+}
+
+// beta (ppm_functs.F90:129-144) on offsets: ut-uc and pt-pc are offset
+// differences; pc/uc/gc absolute values enter only the expansion/gravity terms.
 template <class R, bool GRAV, bool COSMO>
-__device__ __forceinline__ R beta_fn(const SweepScalars<R>& S, R rct, R ut, R uc, R pt, R pc, R gc, R sign)
+__device__ __forceinline__ R beta_d(const SweepScalars<R>& S, R rct, R ut, R uc, R pt, R pc, R pc_abs, R uc_abs,
+                                    R gc_abs, R sign)
 {
     R sum = (ut - uc) + sign * (pt - pc) * rct;
-    if (COSMO) sum = sum + sign * S.dt * (R(2) * S.dat * pc * rct / S.at);
+    if (COSMO) sum = sum + sign * S.dt * (R(2) * S.dat * pc_abs * rct / S.at);
     if (GRAV || COSMO) {
-        R gvc = ((GRAV ? -gc : R(0)) + (COSMO ? S.dat * uc : R(0))) * S.rat;
+        R gvc = ((GRAV ? -gc_abs : R(0)) + (COSMO ? S.dat * uc_abs : R(0))) * S.rat;
         sum = sum + S.dt * gvc;
     }
     return -sign * sum * R(0.5) * rct;
 }
 
-template <class R, bool COSMO>
-__device__ __forceinline__ R beta0_fn(const SweepScalars<R>& S, R rct, R rrhot, R rhoc, R pt, R pc)
+// Traced face state of a cell.  RIGHT = true: its right face (the
+// reference's left state of interface c+1/2, fl of the parabola at the
+// characteristics u+c, u, u-c, ppm_functs.F90:59-75, order2interp.F90:192-219);
+// RIGHT = false: its left face (the right state of c-1/2, fr at -(u-c), -u,
+// -(u+c), ppm_functs.F90:62-84, order2interp.F90:221-247).
+template <class R, bool GRAV, bool COSMO, bool RIGHT>
+__device__ __forceinline__ Face<double> trace_face_d(const SweepScalars<R>& S, const Parab<R>* q, const double* a,
+                                                     double cs, double gamma, int& nclamp)
 {
-    R rct2 = rct * rct;
-    R b = (pt - pc) * rct2 + rrhot - rcp(rhoc);
-    if (COSMO) b = b + (R(2) * S.dat * pc) * rct2 / S.at * S.dt;
-    return b;
-}
-
-// right face of cell c (the reference's left state of interface c+1/2)
-template <class R, bool GRAV, bool COSMO>
-__device__ __forceinline__ Face<R> trace_right_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs,
-                                                    int& nclamp)
-{
-    const R k = S.dtrat * S.rdx;           // dt*rat/dx
-    R x3 = (u + cs) * k;
-    R xt = tmax(x3, R(0));
-    R x1 = (u - cs) * k;
-    R x2 = u * k;
-    R ptl = pfl(q[VP], xt), rtl = pfl(q[VR], xt), utl = pfl(q[VU], xt);
-    R vtl = pfl(q[VV], xt), wtl = pfl(q[VW], xt);
-    R pc1 = pfl(q[VP], x1), pc2 = pfl(q[VP], x2), pc3 = pfl(q[VP], x3);
-    R rc2 = pfl(q[VR], x2), uc1 = pfl(q[VU], x1), uc3 = pfl(q[VU], x3);
-    R vc2 = pfl(q[VV], x2), wc2 = pfl(q[VW], x2);
-    R gc1 = R(0), gc3 = R(0);
-    if (GRAV) { gc1 = pfl(q[VG], x1); gc3 = pfl(q[VG], x3); }
-    if (ptl < R(0)) { ptl = R(K_1EM15); ++nclamp; }
-    if (rtl < R(0)) { rtl = R(K_1EM15); ++nclamp; }
-    R ct = fsqrt(S.gamma * ptl * rtl);
-    R rct = rcp(ct);
-    R rrt = rcp(rtl);
-    const R rat = S.rat;
-    R lm = (u - cs) * rat, l0 = u * rat, lp = (u + cs) * rat;
-    R bm = (lm <= R(0)) ? R(0) : fm::beta_fn<R, GRAV, COSMO>(S, rct, utl, uc1, ptl, pc1, gc1, R(-1));
-    R bp = (lp <= R(0)) ? R(0) : fm::beta_fn<R, GRAV, COSMO>(S, rct, utl, uc3, ptl, pc3, gc3, R(1));
-    R b0 = R(0);
-    R vf = vtl, wf = wtl;
-    if (l0 > R(0)) {
-        b0 = fm::beta0_fn<R, COSMO>(S, rct, rrt, rc2, ptl, pc2);
-        vf = vc2;
-        wf = wc2;
-    }
-    Face<R> f;
-    f.p = ptl + ct * ct * (bp + bm);
-    if (f.p <= R(0)) f.p = ptl;
-    f.u = utl + ct * (bp - bm);
-    f.r = rcp(rrt - bm - b0 - bp);
-    if (f.r <= R(0)) f.r = rtl;
-    f.v = vf;
-    f.w = wf;
-    return f;
-}
-
-// left face of cell c (the reference's right state of interface c-1/2)
-template <class R, bool GRAV, bool COSMO>
-__device__ __forceinline__ Face<R> trace_left_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs,
-                                                   int& nclamp)
-{
+    const double um = a[VU] - cs, u0 = a[VU], up = a[VU] + cs;     // characteristic speeds (FP64 signs)
     const R k = S.dtrat * S.rdx;
-    R x1 = -(u - cs) * k;
-    R xt = tmax(x1, R(0));
-    R x2 = -u * k;
-    R x3 = -(u + cs) * k;
-    R ptr = pfr(q[VP], xt), rtr = pfr(q[VR], xt), utr = pfr(q[VU], xt);
-    R vtr = pfr(q[VV], xt), wtr = pfr(q[VW], xt);
-    R pc1 = pfr(q[VP], x1), pc2 = pfr(q[VP], x2), pc3 = pfr(q[VP], x3);
-    R rc2 = pfr(q[VR], x2), uc1 = pfr(q[VU], x1), uc3 = pfr(q[VU], x3);
-    R vc2 = pfr(q[VV], x2), wc2 = pfr(q[VW], x2);
-    R gc1 = R(0), gc3 = R(0);
-    if (GRAV) { gc1 = pfr(q[VG], x1); gc3 = pfr(q[VG], x3); }
-    if (ptr < R(0)) { ptr = R(K_1EM15); ++nclamp; }
-    if (rtr < R(0)) { rtr = R(K_1EM15); ++nclamp; }
-    R ct = fsqrt(S.gamma * ptr * rtr);
-    R rct = rcp(ct);
-    R rrt = rcp(rtr);
-    const R rat = S.rat;
-    R lm = (u - cs) * rat, l0 = u * rat, lp = (u + cs) * rat;
-    R bp = (lp >= R(0)) ? R(0) : fm::beta_fn<R, GRAV, COSMO>(S, rct, utr, uc3, ptr, pc3, gc3, R(1));
-    R bm = (lm >= R(0)) ? R(0) : fm::beta_fn<R, GRAV, COSMO>(S, rct, utr, uc1, ptr, pc1, gc1, R(-1));
-    R b0 = R(0);
-    R vf = vtr, wf = wtr;
-    if (l0 < R(0)) {
-        b0 = fm::beta0_fn<R, COSMO>(S, rct, rrt, rc2, ptr, pc2);
-        vf = vc2;
-        wf = wc2;
+    R xt, x1, x2, x3;
+    if (RIGHT) {
+        x3 = (R)up * k;
+        xt = tmax(x3, R(0));
+        x1 = (R)um * k;
+        x2 = (R)u0 * k;
+    } else {
+        x1 = -(R)um * k;
+        xt = tmax(x1, R(0));
+        x2 = -(R)u0 * k;
+        x3 = -(R)up * k;
     }
-    Face<R> f;
-    f.p = ptr + ct * ct * (bp + bm);
-    if (f.p <= R(0)) f.p = ptr;
-    f.u = utr + ct * (bp - bm);
-    f.r = rcp(rrt - bm - b0 - bp);
-    if (f.r <= R(0)) f.r = rtr;
-    f.v = vf;
-    f.w = wf;
+    auto tr = [&](const Parab<R>& p, R x) { return RIGHT ? pfl(p, x) : pfr(p, x); };
+    R pt = tr(q[VP], xt), rt = tr(q[VR], xt), ut = tr(q[VU], xt), vt = tr(q[VV], xt), wt = tr(q[VW], xt);
+    const R pc1 = tr(q[VP], x1), pc2 = tr(q[VP], x2), pc3 = tr(q[VP], x3);
+    const R rc2 = tr(q[VR], x2), uc1 = tr(q[VU], x1), uc3 = tr(q[VU], x3);
+    const R vc2 = tr(q[VV], x2), wc2 = tr(q[VW], x2);
+    R gc1 = R(0), gc3 = R(0);
+    if (GRAV) {
+        gc1 = (R)a[VG] + tr(q[VG], x1);
+        gc3 = (R)a[VG] + tr(q[VG], x3);
+    }
+    double ptl = a[VP] + (double)pt, rtl = a[VR] + (double)rt;
+    if (ptl < 0.0) { ptl = K_1EM15; pt = (R)(K_1EM15 - a[VP]); ++nclamp; }
+    if (rtl < 0.0) { rtl = K_1EM15; rt = (R)(K_1EM15 - a[VR]); ++nclamp; }
+    const R ct = fsqrt((R)(gamma * ptl * rtl));
+    const R rct = rcp(ct);
+    const R ap = (R)a[VP], au = (R)a[VU];
+    R bm = R(0), bp = R(0), b0 = R(0);
+    R vf = vt, wf = wt;
+    if (RIGHT) {
+        if (um > 0.0) bm = beta_d<R, GRAV, COSMO>(S, rct, ut, uc1, pt, pc1, ap + pc1, au + uc1, gc1, R(-1));
+        if (up > 0.0) bp = beta_d<R, GRAV, COSMO>(S, rct, ut, uc3, pt, pc3, ap + pc3, au + uc3, gc3, R(1));
+        if (u0 > 0.0) {
+            vf = vc2;
+            wf = wc2;
+        }
+    } else {
+        if (up < 0.0) bp = beta_d<R, GRAV, COSMO>(S, rct, ut, uc3, pt, pc3, ap + pc3, au + uc3, gc3, R(1));
+        if (um < 0.0) bm = beta_d<R, GRAV, COSMO>(S, rct, ut, uc1, pt, pc1, ap + pc1, au + uc1, gc1, R(-1));
+        if (u0 < 0.0) {
+            vf = vc2;
+            wf = wc2;
+        }
+    }
+    if (RIGHT ? (u0 > 0.0) : (u0 < 0.0)) {
+        // beta0 (ppm_functs.F90:149-162) with 1/rho_t - 1/rho_c = (rho_c - rho_t)/(rho_t rho_c)
+        const R rct2 = rct * rct;
+        const R rtr = (R)rtl, rcr = (R)a[VR] + rc2;
+        b0 = (pt - pc2) * rct2 + div(rc2 - rt, rtr * rcr);
+        if (COSMO) b0 = b0 + (R(2) * S.dat * (ap + pc2)) * rct2 / S.at * S.dt;
+    }
+    Face<double> f;
+    f.p = ptl + (double)(ct * ct * (bp + bm));
+    if (f.p <= 0.0) f.p = ptl;
+    f.u = a[VU] + (double)(ut + ct * (bp - bm));
+    // 1/(1/rho_t - bm - b0 - bp) = rho_t / (1 - rho_t (bm + b0 + bp))
+    f.r = rtl * rcp(1.0 - (double)((R)rtl * (bm + b0 + bp)));
+    if (f.r <= 0.0) f.r = rtl;
+    f.v = a[VV] + (double)vf;
+    f.w = a[VW] + (double)wf;
     return f;
+}
+
+// Parabolas of one cell and its two traced faces (delta form).
+//   a[]          the cell's FP64 values (p, rho, u, v, w[, g])
+//   ivl[], ivr[] FP64 interface values before limiting at its left / right interface
+//   rm2..rp2     rho of cells c-2..c+2; drm1/drp1 rho slopes of c-1, c+1
+//   pm1, pp1     p of cells c-1, c+1
+template <class R, int NV, bool GRAV, bool COSMO>
+__device__ __forceinline__ void cell_faces_d(const SweepScalars<R>& S, double gamma, const double* a,
+                                             const double* ivl, const double* ivr, double rm2, double rm1,
+                                             double rp1, double rp2, R drm1, R drp1, double pm1, double pp1,
+                                             bool flat, Face<double>& fL, Face<double>& fR, int& nclamp)
+{
+    const R flatv = flat ? S.flatvalue : R(0);
+    // steepening switch addrho >= addp (order2interp.F90:152-153, ppm_functs.F90:35),
+    // FP64 and cross-multiplied: both minima are positive
+    const bool steep_rho = (0.5 * gamma * fabs(rp1 - rm1)) * tmin(pm1, pp1) >= fabs(pp1 - pm1) * tmin(rm1, rp1);
+    Parab<R> q[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        R l = (R)(ivl[v] - a[v]), r = (R)(ivr[v] - a[v]);
+        if (v == VR && steep_rho) steepen_d(rm2, rm1, a[VR], rp1, rp2, drm1, drp1, l, r);
+        q[v] = limit_parab_d(l, r, flatv);
+    }
+    const double cs = fsqrt(gamma * a[VP] * rcp(a[VR]));        // ppmsolver.F90:109
+    fR = trace_face_d<R, GRAV, COSMO, true>(S, q, a, cs, gamma, nclamp);
+    fL = trace_face_d<R, GRAV, COSMO, false>(S, q, a, cs, gamma, nclamp);
 }
 
 // ---- Riemann solver ------------------------------------------------------------------
@@ -317,7 +395,7 @@ __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vs
             a = cleft * fsqrt(gf * xl + m);
             vpl = -(a * a + cl2) * 0.5 * rcp(a * a * a);
         } else {
-            const double w = pow_pos(xl, m);
+            const double w = pow_m(Rs, xl);
             a = (w == 1.0) ? cleft * fsqrt(gf * xl + m) : m * cleft * div(1.0 - xl, 1.0 - w);
             vpl = -rcl * div(w, xl);
         }
@@ -325,7 +403,7 @@ __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vs
             b = cright * fsqrt(gf * xr + m);
             vpr = (b * b + cr2) * 0.5 * rcp(b * b * b);
         } else {
-            const double w = pow_pos(xr, m);
+            const double w = pow_m(Rs, xr);
             b = (w == 1.0) ? cright * fsqrt(gf * xr + m) : m * cright * div(1.0 - xr, 1.0 - w);
             vpr = rcr * div(w, xr);
         }
@@ -356,10 +434,11 @@ template <class R> __device__ __forceinline__ R vshock(R rs, R vxm, R rho, R vx)
     return (rs == rho) ? R(0) : div(rs * vxm - rho * vx, rs - rho);
 }
 
-// riemannsolver.F90:32-285.  Non-Newton arithmetic in R, Newton in FP64.
-template <class R>
-__device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<R>& L, const Face<R>& Rt, int& nit)
+// riemannsolver.F90:32-285, FP64 throughout (the mixed form keeps the Riemann
+// problem in FP64: its star state feeds the fluxes directly).
+__device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double>& L, const Face<double>& Rt, int& nit)
 {
+    using R = double;
     const R gamma = (R)Rs.gamma, m = (R)Rs.m, rm = (R)Rs.rm, rgamma = (R)Rs.rgamma, rgamma1 = (R)Rs.rgamma1;
     const R g1 = (R)Rs.gamma1;
     const R pleft = L.p, pright = Rt.p, rholeft = L.r, rhoright = Rt.r;
@@ -377,7 +456,7 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<R>& L,
     int kase;
     if (dvv >= dv2s) kase = 1;
     else {
-        const R dv1s = -(R(2) * c2 * rgamma1) * (R(1) - pow_pos(div(p1, p2), m));
+        const R dv1s = -(R(2) * c2 * rgamma1) * (R(1) - pow_m(Rs, div(p1, p2)));
         kase = (dvv >= dv1s) ? 2 : (dvv > dv2r ? 3 : 4);
     }
     if (kase == 1 || kase == 2) {
@@ -385,9 +464,8 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<R>& L,
         if (!fm::pcheck_guess(Rs, pleft, pright, rholeft, rhoright, vxleft, vxright, pmd)) {
             if (kase == 1) pmd = (double)tmax(pleft, pright);        // riemannsolver.F90:77
             else                                                      // :117-119
-                pmd = pow_pos((double)div(cleft + cright + (vxleft - vxright) * (gamma - R(1)) * R(0.5),
-                                          div(cleft, pow_pos(pleft, m)) + div(cright, pow_pos(pright, m))),
-                              Rs.rm);
+                pmd = pow_rm(Rs, div(cleft + cright + (vxleft - vxright) * (gamma - R(1)) * R(0.5),
+                                     div(cleft, pow_m(Rs, pleft)) + div(cright, pow_m(Rs, pright))));
         }
         nit = fm::newton(Rs, pmd, vxmd, pleft, pright, rholeft, rhoright, vxleft, vxright);
         pm = (R)pmd;
@@ -403,7 +481,7 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<R>& L,
             else { pm = pleft; vxm = vxleft; rhom = rholeft; }
         } else if (pleft >= pright) {
             const R rs2 = rhoshock(g1, pm, pright, rhoright);
-            const R rc1 = rholeft * pow_pos(div(pm, pleft), rgamma);
+            const R rc1 = rholeft * pow_rgamma(Rs, div(pm, pleft));
             const R vs2 = vshock(rs2, vxm, rhoright, vxright);
             const R vtail1 = vxm - fsqrt(gamma * pm * rcp(rc1));
             const R vhead1 = vxleft - cleft;
@@ -418,7 +496,7 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<R>& L,
             }
         } else {
             const R rs1 = rhoshock(g1, pm, pleft, rholeft);
-            const R rc2 = rhoright * pow_pos(div(pm, pright), rgamma);
+            const R rc2 = rhoright * pow_rgamma(Rs, div(pm, pright));
             const R vs1 = vshock(rs1, vxm, rholeft, vxleft);
             const R vtail2 = vxm + fsqrt(gamma * pm * rcp(rc2));
             const R vhead2 = vxright + cright;
@@ -433,13 +511,13 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<R>& L,
             }
         }
     } else if (kase == 3) {                               // two rarefactions (no Newton)
-        const R zeta = cright * rcp(cleft) * pow_pos(div(pleft, pright), m);
+        const R zeta = cright * rcp(cleft) * pow_m(Rs, div(pleft, pright));
         const R vtleft = vxleft + R(2) * cleft * rgamma1;
         const R vtright = vxright - R(2) * cright * rgamma1;
         vxm = div(vtleft * zeta + vtright, R(1) + zeta);
-        pm = pleft * pow_pos(div((gamma - R(1)) * (vtleft - vtright), R(2) * cleft * (R(1) + zeta)), rm);
-        const R rc1 = rholeft * pow_pos(div(pm, pleft), rgamma);
-        const R rc2 = rhoright * pow_pos(div(pm, pright), rgamma);
+        pm = pleft * pow_rm(Rs, div((gamma - R(1)) * (vtleft - vtright), R(2) * cleft * (R(1) + zeta)));
+        const R rc1 = rholeft * pow_rgamma(Rs, div(pm, pleft));
+        const R rc2 = rhoright * pow_rgamma(Rs, div(pm, pright));
         const R vtail2 = vxm + fsqrt(gamma * pm * rcp(rc2));
         const R vhead2 = vxright + cright;
         const R vtail1 = vxm - fsqrt(gamma * pm * rcp(rc1));

@@ -21,33 +21,29 @@
 
 namespace hp {
 
-// Reference-order arithmetic (hp_core.h).
+// Both policies present the same interface to the kernels: cell values,
+// interface values and face states are FP64; slopes are of type R (double in
+// the FP64 forms, float in the mixed form).
+template <class R> struct AllScalars;
+
+// Reference-order arithmetic (hp_core.h); R must be double.
 struct ExactMath {
-    template <class R> static __device__ __forceinline__ R slope(R am1, R a, R ap1) { return ajside(am1, a, ap1); }
-    template <class R> static __device__ __forceinline__ R iface(R am1, R a0, R dm1, R d0)
+    template <class R> static __device__ __forceinline__ R slope(double am1, double a, double ap1)
     {
-        return iface_value(am1, a0, dm1, d0);
+        return ajside(am1, a, ap1);
     }
-    template <class R> static __device__ __forceinline__ bool steep(R a0, R a1, R thr) { return pair_steep(a0, a1, thr); }
-    template <class R>
-    static __device__ __forceinline__ void steepen_(R a1, R a2, R a3, R a4, R a5, R dml, R dmr, R dx, R& l, R& r)
+    template <class R> static __device__ __forceinline__ double iface(double am1, double a0, R dm1, R d0)
     {
-        steepen(a1, a2, a3, a4, a5, dml, dmr, dx, l, r);
+        return iface_value<double>(am1, a0, dm1, d0);
     }
-    template <class R> static __device__ __forceinline__ R addratio(R num, R den) { return num / den; }
-    template <class R> static __device__ __forceinline__ R sound(R gamma, R p, R rho) { return sqrt(gamma * p / rho); }
-    template <class R, bool G, bool C>
-    static __device__ __forceinline__ Face<R> right_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs, int& n)
-    {
-        return trace_right_face<R, G, C>(S, q, u, cs, n);
-    }
-    template <class R, bool G, bool C>
-    static __device__ __forceinline__ Face<R> left_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs, int& n)
-    {
-        return trace_left_face<R, G, C>(S, q, u, cs, n);
-    }
-    template <class R>
-    static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<R>& L, const Face<R>& Rt, int& nit)
+    static __device__ __forceinline__ bool steep(double a0, double a1, double thr) { return pair_steep(a0, a1, thr); }
+    template <class R, int NV, bool GRAV, bool COSMO>
+    static __device__ __forceinline__ void cell_faces(const AllScalars<R>& A, const double* a, const double* ivl,
+                                                      const double* ivr, double rm2, double rm1, double rp1,
+                                                      double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
+                                                      Face<double>& fL, Face<double>& fR, int& nclamp);
+    static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
+                                                   int& nit)
     {
         return riemann_solve(Rs, L, Rt, nit);
     }
@@ -60,36 +56,24 @@ struct ExactMath {
     static __device__ __forceinline__ double vsound(double gamma, double p, double rho) { return sqrt(gamma * p / rho); }
 };
 
-// Throughput forms (hp_fast.h).
+// Throughput forms (hp_fast.h), delta-form reconstruction.
 struct FastMath {
-    template <class R> static __device__ __forceinline__ R slope(R am1, R a, R ap1) { return fm::ajside(am1, a, ap1); }
-    template <class R> static __device__ __forceinline__ R iface(R am1, R a0, R dm1, R d0)
+    template <class R> static __device__ __forceinline__ R slope(double am1, double a, double ap1)
     {
-        return fm::iface_value(am1, a0, dm1, d0);
+        return fm::slope_d<R>(am1, a, ap1);
     }
-    template <class R> static __device__ __forceinline__ bool steep(R a0, R a1, R thr) { return fm::pair_steep(a0, a1, thr); }
-    template <class R>
-    static __device__ __forceinline__ void steepen_(R a1, R a2, R a3, R a4, R a5, R dml, R dmr, R, R& l, R& r)
+    template <class R> static __device__ __forceinline__ double iface(double am1, double a0, R dm1, R d0)
     {
-        fm::steepen(a1, a2, a3, a4, a5, dml, dmr, l, r);
+        return fm::iface_d<R>(am1, a0, dm1, d0);
     }
-    template <class R> static __device__ __forceinline__ R addratio(R num, R den) { return fm::div(num, den); }
-    template <class R> static __device__ __forceinline__ R sound(R gamma, R p, R rho)
-    {
-        return fm::fsqrt(gamma * p * fm::rcp(rho));
-    }
-    template <class R, bool G, bool C>
-    static __device__ __forceinline__ Face<R> right_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs, int& n)
-    {
-        return fm::trace_right_face<R, G, C>(S, q, u, cs, n);
-    }
-    template <class R, bool G, bool C>
-    static __device__ __forceinline__ Face<R> left_face(const SweepScalars<R>& S, const Parab<R>* q, R u, R cs, int& n)
-    {
-        return fm::trace_left_face<R, G, C>(S, q, u, cs, n);
-    }
-    template <class R>
-    static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<R>& L, const Face<R>& Rt, int& nit)
+    static __device__ __forceinline__ bool steep(double a0, double a1, double thr) { return fm::pair_steep_d(a0, a1, thr); }
+    template <class R, int NV, bool GRAV, bool COSMO>
+    static __device__ __forceinline__ void cell_faces(const AllScalars<R>& A, const double* a, const double* ivl,
+                                                      const double* ivr, double rm2, double rm1, double rp1,
+                                                      double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
+                                                      Face<double>& fL, Face<double>& fR, int& nclamp);
+    static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
+                                                   int& nit)
     {
         return fm::riemann_solve(Rs, L, Rt, nit);
     }
@@ -180,35 +164,48 @@ template <class R> __device__ __forceinline__ AllScalars<R> load_scalars(const P
     A.Rs.m = P.m; A.Rs.rm = P.rm; A.Rs.gf = P.gf;
     A.Rs.dadt = 1.0 - 0.5 * P.dat * P.dt * P.rat;     // riemannsolver.F90:36
     A.Rs.dmax = P.dmax;
+    A.Rs.m5 = fabs(P.m - 0.2) < 1e-15;
     A.Is.rdtath = P.rdtath; A.Is.rdx = P.rdx; A.Is.gamma = P.gamma; A.Is.rgamma1 = P.rgamma1;
     A.Is.eta1 = P.eta1; A.Is.eta2 = P.eta2;
     A.rgamma1 = P.rgamma1; A.gamma = P.gamma; A.mingradflat = P.mingradflat;
     return A;
 }
 
-// Parabolas of one cell and its two traced face states.
+// Parabolas of one cell and its two traced face states, reference order.
 //   a[]            the cell's values (p, rho, u, v, w[, g])
 //   ivl[], ivr[]   interface values before limiting at its left / right interface
 //   rm2..rp2       rho of cells c-2..c+2 (steepening), drm1/drp1 rho slopes of c-1, c+1
 //   pm1, pp1       p of cells c-1, c+1
-template <class M, class R, int NV, bool GRAV, bool COSMO>
-__device__ __forceinline__ void cell_faces(const SweepScalars<R>& S, const R* a, const R* ivl, const R* ivr,
-                                           R rm2, R rm1, R rp1, R rp2, R drm1, R drp1, R pm1, R pp1, bool flat,
-                                           Face<R>& fL, Face<R>& fR, int& nclamp)
+template <class R, int NV, bool GRAV, bool COSMO>
+__device__ __forceinline__ void ExactMath::cell_faces(const AllScalars<R>& A, const double* a, const double* ivl,
+                                                      const double* ivr, double rm2, double rm1, double rp1,
+                                                      double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
+                                                      Face<double>& fL, Face<double>& fR, int& nclamp)
 {
-    const R flatv = flat ? S.flatvalue : R(0);
-    const R addp = M::addratio(fabs(pp1 - pm1), tmin(pm1, pp1));              // order2interp.F90:152
-    const R addrho = M::addratio(R(0.5) * S.gamma * fabs(rp1 - rm1), tmin(rm1, rp1));   // :153
-    Parab<R> q[NV];
+    const SweepScalars<double>& S = A.S;
+    const double flatv = flat ? S.flatvalue : 0.0;
+    const double addp = fabs(pp1 - pm1) / tmin(pm1, pp1);                          // order2interp.F90:152
+    const double addrho = 0.50 * S.gamma * fabs(rp1 - rm1) / tmin(rm1, rp1);       // :153
+    Parab<double> q[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        R ajl = ivl[v], ajr = ivr[v];
-        if (v == VR && addrho >= addp) M::steepen_(rm2, rm1, a[VR], rp1, rp2, drm1, drp1, S.dx, ajl, ajr);
+        double ajl = ivl[v], ajr = ivr[v];
+        if (v == VR && addrho >= addp) steepen(rm2, rm1, a[VR], rp1, rp2, drm1, drp1, S.dx, ajl, ajr);
         q[v] = limit_parab(a[v], ajl, ajr, flatv);
     }
-    const R cs = M::sound(S.gamma, a[VP], a[VR]);                             // ppmsolver.F90:109
-    fR = M::template right_face<R, GRAV, COSMO>(S, q, a[VU], cs, nclamp);
-    fL = M::template left_face<R, GRAV, COSMO>(S, q, a[VU], cs, nclamp);
+    const double cs = sqrt(S.gamma * a[VP] / a[VR]);                               // ppmsolver.F90:109
+    fR = trace_right_face<double, GRAV, COSMO>(S, q, a[VU], cs, nclamp);
+    fL = trace_left_face<double, GRAV, COSMO>(S, q, a[VU], cs, nclamp);
+}
+
+template <class R, int NV, bool GRAV, bool COSMO>
+__device__ __forceinline__ void FastMath::cell_faces(const AllScalars<R>& A, const double* a, const double* ivl,
+                                                     const double* ivr, double rm2, double rm1, double rp1,
+                                                     double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
+                                                     Face<double>& fL, Face<double>& fR, int& nclamp)
+{
+    fm::cell_faces_d<R, NV, GRAV, COSMO>(A.S, A.gamma, a, ivl, ivr, rm2, rm1, rp1, rp2, drm1, drp1, pm1, pp1, flat,
+                                         fL, fR, nclamp);
 }
 
 // Store one updated cell and, for the axes in `mask`, its periodic face-ghost
@@ -326,19 +323,19 @@ __global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, i
             load(qn, nxt);
         }
 
-        R a[NV], am1[NV], ap1[NV], d[NV], dm1[NV], iv[NV], ivr[NV];
+        double am1[NV], ap1[NV], iv[NV], ivr[NV];
+        R d[NV], dm1[NV];
 #pragma unroll
         for (int s = 0; s < NV; ++s) {
-            a[s] = (R)cd[s];
-            am1[s] = from_left(a[s]);
-            ap1[s] = from_right(a[s]);
+            am1[s] = from_left(cd[s]);
+            ap1[s] = from_right(cd[s]);
         }
 #pragma unroll
-        for (int s = 0; s < NV; ++s) d[s] = M::slope(am1[s], a[s], ap1[s]);   // slope of this cell
+        for (int s = 0; s < NV; ++s) d[s] = M::template slope<R>(am1[s], cd[s], ap1[s]);   // slope of this cell
 #pragma unroll
         for (int s = 0; s < NV; ++s) {
             dm1[s] = from_left(d[s]);
-            iv[s] = M::iface(am1[s], a[s], dm1[s], d[s]);                // interface c-1/2
+            iv[s] = M::iface(am1[s], cd[s], dm1[s], d[s]);                // interface c-1/2
         }
 #pragma unroll
         for (int s = 0; s < NV; ++s) ivr[s] = from_right(iv[s]);          // interface c+1/2
@@ -346,31 +343,31 @@ __global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, i
         // flattening: pair bit of (c-1,c), OR over pairs c-2..c+2 via the wave mask
         bool bpair = false;
 #pragma unroll
-        for (int s = 0; s < 5; ++s) bpair |= M::steep(am1[s], a[s], (R)A.mingradflat);
+        for (int s = 0; s < 5; ++s) bpair |= M::steep(am1[s], cd[s], A.mingradflat);
         const uint64_t Mb = __ballot(bpair);
         const uint64_t Mf = (Mb << 1) | Mb | (Mb >> 1) | (Mb >> 2);
         const bool flat = (A.mingradflat <= 0.0) || ((Mf >> lane) & 1ull);
 
-        const R rm2 = from_left(am1[VR]);
-        const R rp2 = from_right(ap1[VR]);
+        const double rm2 = from_left(am1[VR]);
+        const double rp2 = from_right(ap1[VR]);
         const R drp1 = from_right(d[VR]);
         int ncl = 0;
-        Face<R> fL, fR;
-        cell_faces<M, R, NV, GRAV, COSMO>(A.S, a, iv, ivr, rm2, am1[VR], ap1[VR], rp2, dm1[VR], drp1, am1[VP],
-                                          ap1[VP], flat, fL, fR, ncl);
+        Face<double> fL, fR;
+        M::template cell_faces<R, NV, GRAV, COSMO>(A, cd, iv, ivr, rm2, am1[VR], ap1[VR], rp2, dm1[VR], drp1,
+                                                   am1[VP], ap1[VP], flat, fL, fR, ncl);
 
         // Riemann problem at interface c-1/2: right face of c-1 | left face of c
-        const Face<R> fRm1 = from_left(fR);
+        const Face<double> fRm1 = from_left(fR);
         int nit;
         Flux F = M::riemann(A.Rs, fRm1, fL, nit);
         // high-density-peak floor (riemannsolver.F90:279-285), needs rho c-3..c
-        const uint64_t M1 = __ballot((double)a[VR] > A.Rs.dmax * (double)am1[VR]);
-        const uint64_t M2 = __ballot((double)am1[VR] > A.Rs.dmax * (double)a[VR]);
+        const uint64_t M1 = __ballot(cd[VR] > A.Rs.dmax * am1[VR]);
+        const uint64_t M2 = __ballot(am1[VR] > A.Rs.dmax * cd[VR]);
         const uint64_t Mc = (M1 << 2) & M2;
         if (Mc) {
-            const double r2 = (double)rm2, r1 = (double)from_left(rm2);
-            const double u3 = (double)am1[VU], u2 = (double)from_left(am1[VU]);
-            if ((Mc >> lane) & 1ull) peak_floor(A.Rs, F, r1, r2, (double)am1[VR], (double)a[VR], u2, u3);
+            const double r1 = from_left(rm2);
+            const double u3 = am1[VU], u2 = from_left(am1[VU]);
+            if ((Mc >> lane) & 1ull) peak_floor(A.Rs, F, r1, rm2, am1[VR], cd[VR], u2, u3);
         }
         const Flux F2 = from_right(F);                                    // interface c+1/2
 
@@ -417,19 +414,18 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, i
     const int s1 = min(s0 + seglen, Nax);
     const long base = (nb + ia) + (long)(nb + ib) * s_b + (long)nb * s_ax;   // interior cell 0 of the pencil
     const AllScalars<R> A = load_scalars<R>(B.prm);
-    const R thr = (R)A.mingradflat;
     const bool flat_always = A.mingradflat <= 0.0;
 
     double w0[NV], w1[NV], w2[NV], w3[NV];   // cells k-4 .. k-1
     double r5 = 0.0;                         // rho of cell k-5
     R d1[NV], d2[NV];                        // slopes of cells k-3, k-2
-    R iv2[NV];                               // interface value at k-2 (left of cell k-2)
+    double iv2[NV];                          // interface value at k-2 (left of cell k-2)
     unsigned bits = 0;                       // pair bits of k-3, k-2, k-1
-    Face<R> fRprev;                          // right face of cell k-3
+    Face<double> fRprev;                     // right face of cell k-3
     Flux Fprev;                              // flux at interface k-3 (left of cell k-3)
 #pragma unroll
-    for (int s = 0; s < NV; ++s) { w0[s] = w1[s] = w2[s] = w3[s] = 0.0; d1[s] = d2[s] = iv2[s] = R(0); }
-    fRprev.p = fRprev.r = fRprev.u = fRprev.v = fRprev.w = R(0);
+    for (int s = 0; s < NV; ++s) { w0[s] = w1[s] = w2[s] = w3[s] = iv2[s] = 0.0; d1[s] = d2[s] = R(0); }
+    fRprev.p = fRprev.r = fRprev.u = fRprev.v = fRprev.w = 0.0;
     Fprev.p = Fprev.r = Fprev.u = Fprev.v = Fprev.w = 0.0;
     int nclamp = 0, nitmax = 0, nonconv = 0;
     double vmax = 0.0;
@@ -454,30 +450,28 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, i
                 if (GRAV) nxt[NV - 1] = -B.gforce[3 * qn + (DIR - 1)];
             }
 
-            R d3[NV], iv1[NV];
+            R d3[NV];
+            double iv1[NV];
             bool bk = false;
             if (kk >= s0 - 1) {
 #pragma unroll
-                for (int s = 0; s < NV; ++s) d3[s] = M::slope((R)w2[s], (R)w3[s], (R)cur[s]);
+                for (int s = 0; s < NV; ++s) d3[s] = M::template slope<R>(w2[s], w3[s], cur[s]);
             } else {
 #pragma unroll
                 for (int s = 0; s < NV; ++s) d3[s] = R(0);
             }
             if (kk >= s0 - 2) {
 #pragma unroll
-                for (int s = 0; s < 5; ++s) bk |= M::steep((R)w3[s], (R)cur[s], thr);
+                for (int s = 0; s < 5; ++s) bk |= M::steep(w3[s], cur[s], A.mingradflat);
             }
 #pragma unroll
-            for (int s = 0; s < NV; ++s) iv1[s] = M::iface((R)w2[s], (R)w3[s], d2[s], d3[s]);
+            for (int s = 0; s < NV; ++s) iv1[s] = M::iface(w2[s], w3[s], d2[s], d3[s]);
 
-            Face<R> fL = fRprev, fR = fRprev;
+            Face<double> fL = fRprev, fR = fRprev;
             if (kk >= s0 + 1) {     // parabola and faces of cell k-2
                 const bool flat = flat_always || bits != 0u || bk;
-                R a[NV];
-#pragma unroll
-                for (int s = 0; s < NV; ++s) a[s] = (R)w2[s];
-                cell_faces<M, R, NV, GRAV, COSMO>(A.S, a, iv2, iv1, (R)w0[VR], (R)w1[VR], (R)w3[VR], (R)cur[VR],
-                                               d1[VR], d3[VR], (R)w1[VP], (R)w3[VP], flat, fL, fR, nclamp);
+                M::template cell_faces<R, NV, GRAV, COSMO>(A, w2, iv2, iv1, w0[VR], w1[VR], w3[VR], cur[VR], d1[VR],
+                                                           d3[VR], w1[VP], w3[VP], flat, fL, fR, nclamp);
             }
             Flux F = Fprev;
             if (kk >= s0 + 2) {     // Riemann problem at interface k-2
