@@ -62,7 +62,44 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("linking libhydropad_amd.so failed")
     os.replace(tmp, OUT)
+    build_fortran(verbose)
     return OUT
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "hydropad_driver")
+
+
+def flang():
+    for c in ("/opt/rocm/llvm/bin/flang", shutil.which("flang") or ""):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose=False):
+    """Compile the ISO_C_BINDING module (fortran/hydropad_gpu.F90) and link the
+    evolve_sys-shaped driver against libhydropad_amd.so.  Skipped, with a
+    message, when no Fortran compiler is installed."""
+    fc = flang()
+    if fc is None:
+        sys.stderr.write("build: no flang; skipping the Fortran module and driver\n")
+        return None
+    objdir = os.path.join(HERE, "_build")
+    mod_o = os.path.join(objdir, "hydropad_gpu_f.o")
+    steps = [[fc, "-O2", "-fPIC", "-module-dir", objdir, "-c", os.path.join(FORTRAN, "hydropad_gpu.F90"),
+              "-o", mod_o],
+             [fc, "-O2", "-module-dir", objdir, os.path.join(FORTRAN, "hydropad_driver.F90"), mod_o,
+              "-L" + HERE, "-lhydropad_amd", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"]]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("flang failed: " + " ".join(cmd))
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

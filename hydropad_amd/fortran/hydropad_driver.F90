@@ -1,0 +1,129 @@
+! hydropad_driver.F90 -- a time loop shaped like hydropad/main.F90:100-121 and
+! hydropad/evolve_sys.F90 (nrot schedule, ppm_lxlylz, speed, timedet) over
+! the GPU hydro step, driven through the ISO_C_BINDING module.  Sets up the
+! planar Sod shock tube (rho,p = 1,1 for cell centre x < 0.5, else 0.125,0.1),
+! runs it, and writes the final fields in the reference's dump format
+! (hydropad/main.F90:125-136 + io/write_var_mpi.F90: rho.bin, pressure.bin,
+! vx.bin, vy.bin, vz.bin; REAL(4) interior, Fortran order, no header).
+!
+!   hydropad_driver N NSTEPS [fp64|mixed|exact] [OUTDIR]
+program hydropad_driver
+  use iso_c_binding
+  use hydropad_gpu
+  implicit none
+  integer :: n, nsteps, nb, nx, ny, nz, nstep, nrot, i
+  real(c_double), allocatable :: p3d(:,:,:), rho3d(:,:,:), vx3d(:,:,:), vy3d(:,:,:), vz3d(:,:,:)
+  real(c_double) :: gamma, gamma1, rgamma, rgamma1, m, rm, gf, dx, rdx, dt, t, cour, velmax, x, tstop
+  real(c_double) :: flatvalue, mingradflat, eta1, eta2, dmax
+  character(len=256) :: arg, outdir
+
+  n = 64; nsteps = 10; outdir = '.'
+  if (command_argument_count() >= 1) then
+    call get_command_argument(1, arg); read(arg, *) n
+  end if
+  if (command_argument_count() >= 2) then
+    call get_command_argument(2, arg); read(arg, *) nsteps
+  end if
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg)
+    if (trim(arg) == 'mixed') call hydropad_gpu_set_precision(HP_MIXED)
+    if (trim(arg) == 'exact') call hydropad_gpu_set_precision(HP_FP64_EXACT)
+  end if
+  if (command_argument_count() >= 4) call get_command_argument(4, outdir)
+
+  nb = 3
+  nx = n + 2*nb; ny = nx; nz = nx
+  allocate(p3d(nx,ny,nz), rho3d(nx,ny,nz), vx3d(nx,ny,nz), vy3d(nx,ny,nz), vz3d(nx,ny,nz))
+  vx3d = 0.0d0; vy3d = 0.0d0; vz3d = 0.0d0
+  do i = 1, nx
+    x = (dble(i - nb) - 0.5d0) / dble(n)
+    if (x < 0.5d0) then
+      rho3d(i,:,:) = 1.0d0;   p3d(i,:,:) = 1.0d0
+    else
+      rho3d(i,:,:) = 0.125d0; p3d(i,:,:) = 0.1d0
+    end if
+  end do
+  call fill_periodic(p3d); call fill_periodic(rho3d)
+
+  ! derived constants, hydropad/initialize_parameters.F90:23-24,64-69
+  gamma = 5.0d0/3.0d0
+  rgamma = 1.0d0/gamma
+  gamma1 = (gamma+1.0d0)/(gamma-1.0d0)
+  rgamma1 = 1.0d0/(gamma-1)
+  m = (gamma-1.0d0)/(2.0d0*gamma)
+  rm = 1.0d0/m
+  gf = (gamma+1.0d0)/(2.0d0*gamma)
+  dx = 1.0/real(n)
+  rdx = 1.0d0/dx
+  flatvalue = 0.15d0; mingradflat = 0.25d0; eta1 = 1.0d-3; eta2 = 0.1d0; dmax = 2.0d10
+  cour = 0.8d0; tstop = 1.0d30; t = 0.0d0; dt = 0.0d0
+
+  call hydropad_gpu_set_scalars(mingradflat, flatvalue, gamma, 1.0d0, m, rgamma1, rm, rgamma, dmax, &
+                                rdx, dt, dt, dx, 0.0d0, 1.0d0, gamma1, gf, eta1, eta2)
+  call hydropad_gpu_attach(p3d, rho3d, vx3d, vy3d, vz3d, nx, ny, nz, nb)
+  call speed_gpu(velmax)
+  call timedet(velmax)                  ! the first dt from the initial state's CFL condition
+
+  do nstep = 1, nsteps                  ! hydropad/main.F90:100-121
+    nrot = mod(nstep, 6) + 1            ! hydropad/evolve_sys.F90:30-31
+    if (nstep == 1) nrot = 1
+    call hydropad_gpu_set_scalars(mingradflat, flatvalue, gamma, 1.0d0, m, rgamma1, rm, rgamma, dmax, &
+                                  rdx, dt / 1.0d0, dt, dx, 0.0d0, 1.0d0, gamma1, gf, eta1, eta2)
+    call ppm_lxlylz_gpu(nrot)
+    call speed_gpu(velmax)
+    call timedet(velmax)
+  end do
+
+  call hydropad_gpu_detach(p3d, rho3d, vx3d, vy3d, vz3d)
+  call write_var(rho3d, 'rho.bin'); call write_var(p3d, 'pressure.bin')
+  call write_var(vx3d, 'vx.bin'); call write_var(vy3d, 'vy.bin'); call write_var(vz3d, 'vz.bin')
+  write(*, '(a,i0,a,i0,a,es24.16,a,es24.16,a,es24.16)') 'hydropad_driver: N=', n, ' steps=', nsteps, &
+      ' t=', t, ' dt=', dt, ' sum(rho)=', sum(rho3d(nb+1:nx-nb, nb+1:ny-nb, nb+1:nz-nb))
+  call hydropad_gpu_finalize()
+
+contains
+
+  ! hydropad/functions.F90:2-35 (non-cosmological)
+  subroutine timedet(vmax)
+    real(c_double), intent(in) :: vmax
+    real(c_double) :: told, rdtc
+    told = t
+    rdtc = vmax / (cour * dx)
+    if (vmax == 0.0d0) then
+      dt = 0.1d0
+    else
+      dt = 1.0d0 / rdtc
+    end if
+    t = t + dt
+    if (t > tstop) then
+      t = tstop
+      dt = t - told
+    end if
+  end subroutine timedet
+
+  ! periodic ghosts, z then x then y (hydropad/exchange_mesh.F90:47-123)
+  subroutine fill_periodic(a)
+    real(c_double), intent(inout) :: a(nx, ny, nz)
+    integer :: k
+    do k = 1, nb
+      a(:, :, k) = a(:, :, nz - 2*nb + k); a(:, :, nz - nb + k) = a(:, :, k + nb)
+    end do
+    do k = 1, nb
+      a(k, :, :) = a(nx - 2*nb + k, :, :); a(nx - nb + k, :, :) = a(k + nb, :, :)
+    end do
+    do k = 1, nb
+      a(:, k, :) = a(:, ny - 2*nb + k, :); a(:, ny - nb + k, :) = a(:, k + nb, :)
+    end do
+  end subroutine fill_periodic
+
+  ! REAL(4) interior, Fortran order, no header (io/write_var_mpi.F90:55-65 on one rank)
+  subroutine write_var(a, name)
+    real(c_double), intent(in) :: a(nx, ny, nz)
+    character(len=*), intent(in) :: name
+    integer :: u
+    open(newunit=u, file=trim(outdir)//'/'//name, access='stream', form='unformatted', status='replace')
+    write(u) real(a(nb+1:nx-nb, nb+1:ny-nb, nb+1:nz-nb))
+    close(u)
+  end subroutine write_var
+
+end program hydropad_driver

@@ -9,9 +9,69 @@ calibrate FETCH_SIZE / WRITE_SIZE against bytes actually moved (run with
 HSA_ENABLE_SDMA=0 so the copy is a blit kernel the profiler can count).
 """
 import argparse
+import collections
+import sqlite3
 import time
 
 import numpy as np
+
+
+def summarize(paths, copy_bytes=None):
+    """Per-kernel summary of rocprofv3 result databases (kernel trace and/or
+    counter collection).  Returns text; with copy_bytes, FETCH_SIZE and
+    WRITE_SIZE are also given in bytes, scaled by what the same counter read
+    for the calibration copy (the largest __amd_rocclr_copyBuffer dispatch)."""
+    out = []
+    for path in paths:
+        db = sqlite3.connect(path)
+        tabs = {r[0] for r in db.execute("select name from sqlite_master")}
+        out.append(f"## {path}")
+        if "kernels" in tabs and db.execute("select count(*) from kernels").fetchone()[0]:
+            out.append("| kernel | calls | avg ms | total ms | VGPR | AGPR |")
+            out.append("|---|---|---|---|---|---|")
+            agg = collections.defaultdict(list)
+            regs = {}
+            for name, dur, vg, ag in db.execute("select name, duration, vgpr_count, accum_vgpr_count from kernels"):
+                agg[name].append(dur)
+                regs[name] = (vg, ag)
+            for name, d in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+                out.append(f"| {name.split('(')[0][:80]} | {len(d)} | {np.mean(d) / 1e6:.4f} | "
+                           f"{np.sum(d) / 1e6:.3f} | {regs[name][0]} | {regs[name][1]} |")
+        if "counters_collection" in tabs and db.execute("select count(*) from counters_collection").fetchone()[0]:
+            vals = collections.defaultdict(lambda: collections.defaultdict(list))
+            for disp, name, cn, v in db.execute(
+                    "select dispatch_id, kernel_name, counter_name, value from counters_collection"):
+                vals[name.split("(")[0]][cn].append(v)
+            scale = {}
+            if copy_bytes:
+                cp = [k for k in vals if "copyBuffer" in k]
+                for cn in ("FETCH_SIZE", "WRITE_SIZE"):
+                    if cp and cn in vals[cp[0]]:
+                        scale[cn] = copy_bytes / (max(vals[cp[0]][cn]) * 1024.0)
+            cnames = sorted({c for d in vals.values() for c in d})
+            out.append("| kernel | dispatches | " + " | ".join(cnames) + " |")
+            out.append("|---|---|" + "---|" * len(cnames))
+            for name, d in vals.items():
+                if not ("sweep" in name or "copyBuffer" in name or "speed" in name):
+                    continue
+                cells = []
+                for cn in cnames:
+                    if cn not in d:
+                        cells.append("")
+                        continue
+                    v = float(np.max(d[cn])) if "copyBuffer" in name else float(np.mean(d[cn]))
+                    txt = f"{v:.4g}"
+                    if cn in scale:
+                        txt += f" ({v * 1024.0 * scale[cn] / 1e9:.3f} GB)"
+                    cells.append(txt)
+                n = len(next(iter(d.values())))
+                out.append(f"| {name[:80]} | {n} | " + " | ".join(cells) + " |")
+            if scale:
+                out.append("")
+                out.append("calibration: " + ", ".join(f"{k} x{v:.3f} (bytes per KiB read)" for k, v in scale.items())
+                           + f" from a {copy_bytes / 2**30:.0f} GiB device-to-device copy")
+        out.append("")
+    return "\n".join(out)
 
 
 def main():
@@ -20,14 +80,19 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed", "fp64_exact"))
     ap.add_argument("--ic", default="sod", choices=("sod", "random"))
+    ap.add_argument("--axis", type=int, default=0, choices=(0, 1, 2), help="shock-tube axis")
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--summarize", nargs="+", metavar="DB", help="print a summary of rocprofv3 databases and exit")
     args = ap.parse_args()
+    if args.summarize:
+        print(summarize(args.summarize, args.copy_mib * (1 << 20) if args.copy_mib else None))
+        return
 
     import hydropad_amd as hp
     from hydropad_amd import ic
     N = args.size
-    st = ic.sod(N) if args.ic == "sod" else ic.random_smooth(N, 7)
-    print(f"profiling: {args.ic} {N}^3, {args.steps} steps, {args.precision}", flush=True)
+    st = ic.sod(N, axis=args.axis) if args.ic == "sod" else ic.random_smooth(N, 7)
+    print(f"profiling: {args.ic} (axis {args.axis}) {N}^3, {args.steps} steps, {args.precision}", flush=True)
     with hp.Box(N, args.precision) as box:
         box.upload(st)
         dt, t = box.timedet(box.speed(), 0.8)
