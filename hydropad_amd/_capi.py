@@ -44,6 +44,19 @@ class HpCounters(ctypes.Structure):
                 ("neg_state_clamps", ctypes.c_int64), ("reserved", ctypes.c_int64 * 5)]
 
 
+COSMO_NAMES = ("cour", "ca", "tstop", "t", "dt", "at", "dat", "ath", "dath", "atnew", "datnew", "rat", "rdtath")
+
+
+class HpCosmo(ctypes.Structure):
+    """Mirror of hp_cosmo: the Einstein-de Sitter schedule (hydropad/scalar.F90:16-19)."""
+    _fields_ = [(n, _D) for n in COSMO_NAMES] + [("nstep", ctypes.c_int64)]
+
+    def copy(self):
+        c = HpCosmo()
+        ctypes.pointer(c)[0] = self
+        return c
+
+
 class HydropadError(RuntimeError):
     def __init__(self, fn, code):
         super().__init__(f"{fn} failed: {_ERRNAMES.get(code, code)}")
@@ -55,7 +68,8 @@ EXPORTS = ("hp_params_init", "hp_create", "hp_destroy", "hp_set_precision", "hp_
            "hp_download", "hp_upload_device", "hp_download_device", "hp_set_params",
            "hp_ppmsolver", "hp_exchange_mesh", "hp_ppm_lxlylz", "hp_speed", "hp_evolve",
            "hp_status", "hp_reset_counters", "hp_sync", "hp_integrate_cell", "hp_dualenergy_cell",
-           "hp_sweep_timing_enable", "hp_sweep_timing")
+           "hp_sweep_timing_enable", "hp_sweep_timing", "hp_cosmo_init", "hp_timedet_cosmo",
+           "hp_cosmo_params", "hp_savetn", "hp_expand", "hp_evolve_cosmo")
 
 _lib = None
 
@@ -91,6 +105,13 @@ def lib():
     L.hp_dualenergy_cell.argtypes = [PP] + [_D] * 8 + [_PD, _PD]
     L.hp_sweep_timing_enable.argtypes = [_P, _I]
     L.hp_sweep_timing.argtypes = [_P, _PD, ctypes.POINTER(ctypes.c_int64)]
+    PC = ctypes.POINTER(HpCosmo)
+    L.hp_cosmo_init.argtypes = [_D, _D, _D, _D, PC]
+    L.hp_timedet_cosmo.argtypes = [PC, _D, _D]
+    L.hp_cosmo_params.argtypes = [PC, PP]
+    L.hp_savetn.argtypes = [_P]
+    L.hp_expand.argtypes = [_P, _D]
+    L.hp_evolve_cosmo.argtypes = [_P, _I, _I, PC]
     _lib = L
     return L
 
@@ -99,6 +120,19 @@ def check(fn, rc):
     if rc != HP_OK:
         raise HydropadError(fn, rc)
     return rc
+
+
+def cosmo_init(a_start, cour, ca, tstop):
+    """EdS schedule at a_start (hydropad/initialize_parameters.F90:136-178)."""
+    s = HpCosmo()
+    check("hp_cosmo_init", lib().hp_cosmo_init(a_start, cour, ca, tstop, ctypes.byref(s)))
+    return s
+
+
+def timedet_cosmo(s, velmax, dx):
+    """timedet_cosmo (hydropad/functions.F90:38-151) on the host; advances s in place."""
+    check("hp_timedet_cosmo", lib().hp_timedet_cosmo(ctypes.byref(s), velmax, dx))
+    return s
 
 
 def params_init(gamma, ngrid_max, dt=0.0):

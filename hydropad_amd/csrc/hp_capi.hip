@@ -2,6 +2,7 @@
 // state ownership, host<->device copies and the step orchestration of
 // hydro/ppm_lxlylz.F90 and hydropad/evolve_sys.F90 on one stream.
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -10,9 +11,11 @@
 
 #include "../../include/hydropad_gpu.h"
 #include "hp_core.h"
+#include "hp_cosmo.h"
 #include "hp_kernels.h"
 
 static_assert(sizeof(hp_params) == sizeof(hp::Params), "hp_params and hp::Params must match");
+static_assert(sizeof(hp_cosmo) == sizeof(hp::Cosmo), "hp_cosmo and hp::Cosmo must match");
 static_assert(HP_FP64 == hp::PREC_FP64 && HP_MIXED == hp::PREC_MIXED && HP_FP64_EXACT == hp::PREC_FP64_EXACT,
               "precision codes");
 
@@ -68,8 +71,10 @@ struct hp_ctx {
     int precision;
     size_t ncell;
     void* pool;
-    double* f[2][5];
+    void* pool3;           // third field set, allocated on first use (expansion steps)
+    double* f[3][5];       // field sets; sweeps read f[cur] and write another set
     int cur;
+    int saved;             // set holding the kept start-of-step state (hp_savetn), or -1
     double* gforce;
     bool has_grav;
     hp::Params* d_prm;
@@ -98,12 +103,31 @@ static hipEvent_t take_event(hp_ctx* c)
 
 static bool is_cosmo(const hp_params& P) { return P.dat != 0.0 || P.at != 1.0 || P.rat != 1.0; }
 
-static hp::BoxView view_of(hp_ctx* c)
+// The set a sweep from f[cur] writes: neither cur nor the kept state.
+static int out_set(const hp_ctx* c)
+{
+    for (int k = 0; k < 3; ++k)
+        if (k != c->cur && k != c->saved) return k;
+    return -1;
+}
+
+static bool ensure_third_set(hp_ctx* c)
+{
+    if (c->pool3) return true;
+    const size_t align = 256;
+    const size_t fstride = (c->ncell * sizeof(double) + align - 1) / align * align;
+    if (hipMalloc(&c->pool3, 5 * fstride) != hipSuccess) { c->pool3 = nullptr; return false; }
+    char* p = (char*)c->pool3;
+    for (int f = 0; f < 5; ++f) { c->f[2][f] = (double*)p; p += fstride; }
+    return true;
+}
+
+static hp::BoxView view_of(hp_ctx* c, int out)
 {
     hp::BoxView b;
     for (int i = 0; i < 5; ++i) {
         b.in[i] = c->f[c->cur][i];
-        b.out[i] = c->f[1 - c->cur][i];
+        b.out[i] = c->f[out][i];
     }
     b.gforce = c->has_grav ? c->gforce : nullptr;
     b.prm = c->d_prm;
@@ -112,9 +136,16 @@ static hp::BoxView view_of(hp_ctx* c)
     return b;
 }
 
-static int sweep(hp_ctx* c, int dir, int ghost_mask, int do_speed)
+// One sweep f[cur] -> f[out_set]; with expand, the expansion step against the
+// kept state f[saved] with the device factors st->fac is fused into it.
+static int sweep(hp_ctx* c, int dir, int ghost_mask, int do_speed, bool expand = false)
 {
+    const int out = out_set(c);
+    if (out < 0 || (out == 2 && !c->pool3)) return HP_E_STATE;
+    if (expand && c->saved < 0) return HP_E_STATE;
     hp::SweepLaunch L;
+    for (int f = 0; f < 4; ++f) L.old[f] = expand ? c->f[c->saved][f == 0 ? 0 : f + 1] : nullptr;
+    L.fac = (const double*)((const char*)c->d_st + offsetof(hp::DevStatus, fac));
     L.dir = dir;
     L.ghost_mask = ghost_mask;
     L.do_speed = do_speed;
@@ -123,7 +154,7 @@ static int sweep(hp_ctx* c, int dir, int ghost_mask, int do_speed)
     L.precision = c->precision;
     L.grav = c->has_grav;
     L.cosmo = is_cosmo(c->h_prm);
-    hp::BoxView b = view_of(c);
+    hp::BoxView b = view_of(c, out);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         e0 = take_event(c);
@@ -136,7 +167,7 @@ static int sweep(hp_ctx* c, int dir, int ghost_mask, int do_speed)
         c->ev_used[dir - 1].push_back(e0);
         c->ev_used[dir - 1].push_back(e1);
     }
-    c->cur = 1 - c->cur;
+    c->cur = out;
     return HP_OK;
 }
 
@@ -196,6 +227,9 @@ int hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out)
     c->d_prm = (hp::Params*)p; p += align;
     c->d_st = (hp::DevStatus*)p;
     c->cur = 0;
+    c->saved = -1;
+    c->pool3 = nullptr;
+    for (int f = 0; f < 5; ++f) c->f[2][f] = nullptr;
     c->has_grav = false;
     if (hipStreamCreate(&c->stream) != hipSuccess) { hipFree(c->pool); delete c; return HP_E_HIP; }
     hp_params_init(5.0 / 3.0, nx - 2 * nbound, 0.0, &c->h_prm);
@@ -219,6 +253,7 @@ void hp_destroy(hp_ctx* c)
         for (hipEvent_t e : c->ev_used[d]) hipEventDestroy(e);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->pool) hipFree(c->pool);
+    if (c->pool3) hipFree(c->pool3);
     delete c;
 }
 
@@ -235,6 +270,7 @@ static int upload_impl(hp_ctx* c, const double* const src[5], const double* gfor
     for (int f = 0; f < 5; ++f)
         if (!src[f]) return HP_E_ARG;
     const size_t bytes = c->ncell * sizeof(double);
+    c->saved = -1;                               // an upload discards the kept state
     for (int f = 0; f < 5; ++f) HPCHK(hipMemcpyAsync(c->f[c->cur][f], src[f], bytes, kind, c->stream));
     if (gforce) HPCHK(hipMemcpyAsync(c->gforce, gforce, 3 * bytes, kind, c->stream));
     c->has_grav = gforce != nullptr;
@@ -351,6 +387,85 @@ int hp_evolve(hp_ctx* c, int first_step, int nsteps, double cour, double tstop, 
     HPCHK(hipStreamSynchronize(c->stream));
     *t = tt;
     *dt = c->h_prm.dt;
+    return HP_OK;
+}
+
+// ---- comoving boxes ------------------------------------------------------------
+
+int hp_cosmo_init(double a_start, double cour, double ca, double tstop, hp_cosmo* s)
+{
+    if (!s || !(a_start > 0.0) || !(cour > 0.0) || !(ca > 0.0)) return HP_E_ARG;
+    hp::Cosmo c;
+    hp::cosmo_init(a_start, cour, ca, tstop, c);
+    memcpy(s, &c, sizeof(c));
+    return HP_OK;
+}
+
+int hp_timedet_cosmo(hp_cosmo* s, double velmax, double dx)
+{
+    if (!s || !(dx > 0.0) || !(velmax >= 0.0)) return HP_E_ARG;
+    hp::Cosmo c;
+    memcpy(&c, s, sizeof(c));
+    hp::cosmo_timedet(c, velmax, dx);
+    memcpy(s, &c, sizeof(c));
+    return HP_OK;
+}
+
+int hp_cosmo_params(const hp_cosmo* s, hp_params* prm)
+{
+    if (!s || !prm) return HP_E_ARG;
+    prm->at = s->at;
+    prm->dat = s->dat;
+    prm->rat = s->rat;
+    prm->dt = s->dt;
+    prm->rdtath = s->rdtath;
+    return HP_OK;
+}
+
+int hp_savetn(hp_ctx* c)
+{
+    if (!c) return HP_E_ARG;
+    if (!ensure_third_set(c)) return HP_E_NOMEM;
+    c->saved = c->cur;            // sweeps now ping-pong between the other two sets
+    return HP_OK;
+}
+
+int hp_expand(hp_ctx* c, double afact)
+{
+    if (!c) return HP_E_ARG;
+    if (c->saved < 0 || c->saved == c->cur) return HP_E_STATE;
+    const double* old[4] = {c->f[c->saved][0], c->f[c->saved][2], c->f[c->saved][3], c->f[c->saved][4]};
+    HPCHK(hp::launch_expand(c->f[c->cur], old, (long)c->ncell, afact, c->stream));
+    return HP_OK;
+}
+
+int hp_evolve_cosmo(hp_ctx* c, int first_step, int nsteps, hp_cosmo* s)
+{
+    if (!c || !s || nsteps < 0 || first_step < 1) return HP_E_ARG;
+    if (!(s->at > 0.0) || !(s->ath > 0.0) || !(s->cour > 0.0)) return HP_E_ARG;
+    if (!ensure_third_set(c)) return HP_E_NOMEM;
+    hp::Cosmo cs;
+    memcpy(&cs, s, sizeof(cs));
+    hp_cosmo_params(s, &c->h_prm);
+    double fac[3];
+    hp::cosmo_factors(cs, fac);
+    HPCHK(hipMemcpyAsync(c->d_prm, &c->h_prm, sizeof(hp_params), hipMemcpyHostToDevice, c->stream));
+    HPCHK(hipMemcpyAsync(&c->d_st->cosmo, &cs, sizeof(cs), hipMemcpyHostToDevice, c->stream));
+    HPCHK(hipMemcpyAsync(&c->d_st->fac[0], fac, sizeof(fac), hipMemcpyHostToDevice, c->stream));
+    HPCHK(hp::launch_zero_status(c->d_st, 1, c->stream));
+    for (int nstep = first_step; nstep < first_step + nsteps; ++nstep) {
+        const int nrot = (nstep == 1) ? 1 : nstep % 6 + 1;          // evolve_sys.F90:30-31
+        c->saved = c->cur;                                         // start-of-step state
+        for (int k = 0; k < 3; ++k) {
+            int r = sweep(c, NROT_ORDER[nrot - 1][k], 7, k == 2 ? 1 : 0, k == 2);
+            if (r) return r;
+        }
+        HPCHK(hp::launch_timedet_cosmo(c->d_prm, c->d_st, c->stream));
+    }
+    HPCHK(hipMemcpyAsync(&c->h_prm, c->d_prm, sizeof(hp_params), hipMemcpyDeviceToHost, c->stream));
+    HPCHK(hipMemcpyAsync(&cs, &c->d_st->cosmo, sizeof(cs), hipMemcpyDeviceToHost, c->stream));
+    HPCHK(hipStreamSynchronize(c->stream));
+    memcpy(s, &cs, sizeof(cs));
     return HP_OK;
 }
 

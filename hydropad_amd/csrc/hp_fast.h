@@ -134,16 +134,27 @@ __device__ __forceinline__ float pow_pos(float x, float y)
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
 }
 
-// x^(1/5) for x > 0: FP32 seed, then two Newton steps on y^5 = x,
-// y <- (4y + x/y^4)/5 (each step squares the relative error).
+// x^(1/5) for x >= 0: FP32 seed, then two Newton steps on y^5 = x,
+// y <- (4y + x/y^4)/5 (each step squares the relative error).  The seed needs
+// x inside FP32's normal range; near-vacuum rarefactions produce pressure
+// ratios far below it (p*/p ~ 1e-45 and less), so outside [1e-30, 1e30] the
+// argument is first scaled by 2^(-5q) and the root by 2^q.  x = 0 gives 0.
 __device__ __forceinline__ double root5(double x)
 {
-    double y = (double)__builtin_amdgcn_exp2f(0.2f * __builtin_amdgcn_logf((float)x));
+    double xs = x, sc = 1.0;
+    if (!(x > 1e-30 && x < 1e30)) {                 // rare: wave-uniformly skipped
+        if (!(x > 0.0)) return 0.0;
+        const int e = __builtin_amdgcn_frexp_exp(x);
+        const int q = (e >= 0) ? e / 5 : -((4 - e) / 5);
+        xs = __builtin_amdgcn_ldexp(x, -5 * q);
+        sc = __builtin_amdgcn_ldexp(1.0, q);
+    }
+    double y = (double)__builtin_amdgcn_exp2f(0.2f * __builtin_amdgcn_logf((float)xs));
     double y2 = y * y;
-    y = fma(0.8, y, 0.2 * x * rcp(y2 * y2));
+    y = fma(0.8, y, 0.2 * xs * rcp(y2 * y2));
     y2 = y * y;
-    y = fma(0.8, y, 0.2 * x * rcp(y2 * y2));
-    return y;
+    y = fma(0.8, y, 0.2 * xs * rcp(y2 * y2));
+    return y * sc;
 }
 
 // The three powers the Riemann solver takes: x^m, x^(1/gamma), x^(1/m).  For

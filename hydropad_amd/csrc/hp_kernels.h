@@ -8,6 +8,17 @@ namespace hp {
 
 struct Params;
 
+// Einstein-de Sitter time-step schedule of a cosmological box
+// (hydropad/functions.F90:38-151, hydropad/initialize_parameters.F90:136-178);
+// same layout as hp_cosmo in include/hydropad_gpu.h.
+struct Cosmo {
+    double cour, ca, tstop;
+    double t, dt;
+    double at, dat, ath, dath, atnew, datnew;
+    double rat, rdtath;
+    long long nstep;
+};
+
 // Device-resident per-box step state and counters.
 struct DevStatus {
     unsigned long long velmax_bits;   // max(|v|+cs) as the bit pattern of a non-negative double
@@ -16,7 +27,8 @@ struct DevStatus {
     unsigned long long newton_nonconv;
     unsigned long long clamps;
     double t;                          // time at the end of the upcoming step (timedet semantics)
-    double pad1[3];
+    double fac[3];                     // expansion factors of the upcoming step (expand.F90:18-20)
+    Cosmo cosmo;                       // schedule of a cosmological box
 };
 
 // One box as seen by a sweep.  Fields are (nx,ny,nz) arrays with pitch nx,
@@ -40,6 +52,11 @@ struct SweepLaunch {
     int block;        // threads per block
     int precision;    // Precision
     bool grav, cosmo;
+    // expansion step fused into this sweep (hydropad/expand.F90:18-41):
+    // start-of-step p, vx, vy, vz, or old[0] == nullptr; fac -> device
+    // (afact, 1/(1+afact), 1/(2+afact))
+    const double* old[4];
+    const double* fac;
 };
 
 hipError_t launch_sweep(const BoxView& b, const SweepLaunch& L, hipStream_t s);
@@ -50,6 +67,15 @@ hipError_t launch_speed(const double* const f[5], int nx, int ny, int nz, int nb
 // timedet (hydropad/functions.F90:2-35) on the device: consumes velmax_bits,
 // writes prm->dt, prm->rdtath, st->t, and zeroes velmax_bits.
 hipError_t launch_timedet(Params* prm, DevStatus* st, double cour, double tstop, hipStream_t s);
+// timedet_cosmo (EdS branch, hydropad/functions.F90:38-151) on the device:
+// consumes velmax_bits, advances st->cosmo, writes prm->{at,dat,rat,dt,rdtath}
+// and st->fac for the next step's expansion, zeroes velmax_bits.
+hipError_t launch_timedet_cosmo(Params* prm, DevStatus* st, hipStream_t s);
+// expand (hydropad/expand.F90:18-41) over the whole array, in place, against
+// the start-of-step p, vx, vy, vz in old[].
+hipError_t launch_expand(double* const f[5], const double* const old[4], long ncell, double afact, hipStream_t s);
+// dst[k][i] = src[k][i] for the n fields listed (a plain streaming copy).
+hipError_t launch_copy(double* const* dst, const double* const* src, int nfield, long ncell, hipStream_t s);
 // what: bit 0 clears the CFL maximum, bit 1 the counters.
 hipError_t launch_zero_status(DevStatus* st, int what, hipStream_t s);
 

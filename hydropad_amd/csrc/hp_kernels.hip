@@ -2,6 +2,7 @@
 // maximum, timedet) and the dispatcher over the sweep variants of
 // hp_kernels_exact.hip / hp_kernels_fast.hip.
 #include "hp_core.h"
+#include "hp_cosmo.h"
 #include "hp_kernels.h"
 
 namespace hp {
@@ -77,6 +78,52 @@ __global__ void timedet_kernel(Params* prm, DevStatus* st, double cour, double t
     st->velmax_bits = 0ull;
 }
 
+// timedet_cosmo on the device; writes the scalars of the next step.
+__global__ void timedet_cosmo_kernel(Params* prm, DevStatus* st)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double velmax = __longlong_as_double((long long)st->velmax_bits);
+    Cosmo c = st->cosmo;
+    cosmo_timedet(c, velmax, prm->dx);
+    st->cosmo = c;
+    prm->at = c.at;
+    prm->dat = c.dat;
+    prm->rat = c.rat;
+    prm->dt = c.dt;
+    prm->rdtath = c.rdtath;
+    cosmo_factors(c, st->fac);
+    st->velmax_bits = 0ull;
+}
+
+// expand.F90:22-38 over the whole array (the reference loops over ghosts too).
+__global__ void __launch_bounds__(256) expand_kernel(double* __restrict__ p, double* __restrict__ vx,
+                                                     double* __restrict__ vy, double* __restrict__ vz,
+                                                     const double* __restrict__ po, const double* __restrict__ vxo,
+                                                     const double* __restrict__ vyo, const double* __restrict__ vzo,
+                                                     long n, double afact, double afact1, double afact2)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        vx[i] = (2.0 * vx[i] - afact * vxo[i]) * afact2;
+        vy[i] = (2.0 * vy[i] - afact * vyo[i]) * afact2;
+        vz[i] = (2.0 * vz[i] - afact * vzo[i]) * afact2;
+        const double ppp = p[i];
+        double pn = (p[i] - afact * po[i]) * afact1;
+        if (pn <= 0.0) pn = (ppp > 0.0) ? ppp : po[i];
+        p[i] = pn;
+    }
+}
+
+struct CopyArgs {
+    double* dst[5];
+    const double* src[5];
+};
+
+__global__ void __launch_bounds__(256) copy_kernel(CopyArgs a, int nfield, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        for (int f = 0; f < nfield; ++f) a.dst[f][i] = a.src[f][i];
+}
+
 // what: bit 0 clears the CFL maximum, bit 1 the counters.
 __global__ void zero_status_kernel(DevStatus* st, int what)
 {
@@ -133,6 +180,39 @@ hipError_t launch_speed(const double* const f[5], int nx, int ny, int nz, int nb
 hipError_t launch_timedet(Params* prm, DevStatus* st, double cour, double tstop, hipStream_t s)
 {
     hipLaunchKernelGGL(timedet_kernel, dim3(1), dim3(64), 0, s, prm, st, cour, tstop);
+    return hipGetLastError();
+}
+
+hipError_t launch_timedet_cosmo(Params* prm, DevStatus* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(timedet_cosmo_kernel, dim3(1), dim3(64), 0, s, prm, st);
+    return hipGetLastError();
+}
+
+static long stream_grid(long n)
+{
+    long grid = (n + 255) / 256;
+    if (grid > 16384) grid = 16384;
+    return grid < 1 ? 1 : grid;
+}
+
+hipError_t launch_expand(double* const f[5], const double* const old[4], long ncell, double afact, hipStream_t s)
+{
+    const double afact1 = 1.0 / (1.0 + afact), afact2 = 1.0 / (2.0 + afact);
+    hipLaunchKernelGGL(expand_kernel, dim3((unsigned)stream_grid(ncell)), dim3(256), 0, s, f[0], f[2], f[3], f[4],
+                       old[0], old[1], old[2], old[3], ncell, afact, afact1, afact2);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(double* const* dst, const double* const* src, int nfield, long ncell, hipStream_t s)
+{
+    if (nfield < 1 || nfield > 5) return hipErrorInvalidValue;
+    CopyArgs a;
+    for (int f = 0; f < 5; ++f) {
+        a.dst[f] = f < nfield ? dst[f] : nullptr;
+        a.src[f] = f < nfield ? src[f] : nullptr;
+    }
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)stream_grid(ncell)), dim3(256), 0, s, a, nfield, ncell);
     return hipGetLastError();
 }
 

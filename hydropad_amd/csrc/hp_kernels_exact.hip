@@ -7,10 +7,7 @@ namespace hp {
 
 hipError_t launch_sweep_exact(const BoxView& b, const SweepLaunch& L, hipStream_t s)
 {
-    if (L.grav) return L.cosmo ? launch_sweep_t<ExactMath, double, true, true>(b, L, s)
-                               : launch_sweep_t<ExactMath, double, true, false>(b, L, s);
-    return L.cosmo ? launch_sweep_t<ExactMath, double, false, true>(b, L, s)
-                   : launch_sweep_t<ExactMath, double, false, false>(b, L, s);
+    return launch_sweep_variants<ExactMath, double>(b, L, s);
 }
 
 }  // namespace hp

@@ -240,10 +240,67 @@ __device__ __forceinline__ void store_cell(double* const* out, long q, const dou
     }
 }
 
-// Wave-level reduction of the per-lane counters, then one atomic per wave.
-__device__ __forceinline__ void flush_counters(DevStatus* st, int nitmax, int nonconv, int nclamp, double vmax,
-                                               bool want_speed)
+// What a sweep does after integrate, besides the store: the expansion step of
+// hydropad/expand.F90:18-41 against the start-of-step state (cosmological
+// boxes, last sweep of a step; old[0] == nullptr when off) and the CFL maximum.
+struct Epilogue {
+    const double* old[4];    // start-of-step p, vx, vy, vz
+    const double* fac;       // device (afact, afact1, afact2), read by load_factors
+    double afact, afact1, afact2;
+    int ghost_mask;          // bit d-1: write the periodic face-ghost images along axis d
+    int do_speed;
+};
+
+// Per-lane tallies, reduced once per wave.
+struct Tally {
+    int nclamp, nitmax, nonconv;
+    double vmax;
+};
+
+// Read the expansion factors of the step from device memory (kernel entry).
+__device__ __forceinline__ void load_factors(Epilogue& E)
 {
+    if (E.old[0]) { E.afact = E.fac[0]; E.afact1 = E.fac[1]; E.afact2 = E.fac[2]; }
+}
+
+__device__ __forceinline__ Tally tally0()
+{
+    Tally t;
+    t.nclamp = 0; t.nitmax = 0; t.nonconv = 0; t.vmax = 0.0;
+    return t;
+}
+
+// expand.F90:22-38 on one cell; val[] is (p, rho, vx, vy, vz) in physical order.
+// afact1 = 1/(1+afact), afact2 = 1/(2+afact) come precomputed (expand.F90:19-20).
+__device__ __forceinline__ void expand_cell(const Epilogue& E, long q, double* val)
+{
+    const double po = E.old[0][q];
+#pragma unroll
+    for (int f = 2; f < 5; ++f) val[f] = (2.0 * val[f] - E.afact * E.old[f - 1][q]) * E.afact2;
+    const double ppp = val[0];
+    double pn = (val[0] - E.afact * po) * E.afact1;
+    if (pn <= 0.0) pn = (ppp > 0.0) ? ppp : po;
+    val[0] = pn;
+}
+
+// The epilogue of one updated cell: expansion, store (+ ghost images), CFL maximum.
+template <class M, bool EXP>
+__device__ __forceinline__ void finish_cell(const BoxView& B, const Epilogue& E, double gamma, long q, double* val,
+                                            int ix, int iy, int iz, Tally& T)
+{
+    if (EXP) expand_cell(E, q, val);
+    store_cell(B.out, q, val, ix, iy, iz, E.ghost_mask, B.nx, B.ny, B.nz, B.nb);
+    if (E.do_speed) {
+        const double vs = M::vsound(gamma, val[0], val[1]);
+        T.vmax = tmax(T.vmax, tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs);
+    }
+}
+
+// Wave-level reduction of the per-lane counters, then one atomic per wave.
+__device__ __forceinline__ void flush_counters(DevStatus* st, const Tally& T, bool want_speed)
+{
+    int nitmax = T.nitmax, nonconv = T.nonconv, nclamp = T.nclamp;
+    double vmax = T.vmax;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         nitmax = max(nitmax, __shfl_xor(nitmax, off, 64));
@@ -267,9 +324,17 @@ __device__ __forceinline__ void count_newton(int nit, int& nitmax, int& nonconv)
 }
 
 // ============================================================================
-// x sweep: lane per cell, DPP neighbours.
-template <class M, class R, bool GRAV, bool COSMO>
-__global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, int do_speed, int chunks_per_wave)
+// x sweep: lane per cell, DPP neighbours.  A wave advances chunks [c0, c1) of
+// the concatenated x-rows of box B, 58 output cells per chunk.
+__device__ __forceinline__ unsigned x_chunks(int nx, int ny, int nz, int nb)
+{
+    const unsigned total = (unsigned)(ny - 2 * nb) * (unsigned)(nz - 2 * nb) * (unsigned)nx;
+    return (total + 57u) / 58u;
+}
+
+template <class M, class R, bool GRAV, bool COSMO, bool EXP>
+__device__ __forceinline__ void x_run(const BoxView& B, const AllScalars<R>& A, const Epilogue& E, unsigned c0,
+                                      unsigned c1, Tally& T)
 {
     using D = Dirmap<1>;
     constexpr int NV = GRAV ? 6 : 5;
@@ -277,13 +342,7 @@ __global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, i
     const int Nx = nx - 2 * nb;
     const unsigned Ny = (unsigned)(ny - 2 * nb);
     const unsigned total = Ny * (unsigned)(B.nz - 2 * nb) * (unsigned)nx;   // concatenated x-rows
-    const unsigned nchunks = (total + 57u) / 58u;
     const int lane = threadIdx.x & 63;
-    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const unsigned c0 = wave * (unsigned)chunks_per_wave;
-    if (c0 >= nchunks) return;                         // whole wave past the end (uniform)
-    const unsigned c1 = min(c0 + (unsigned)chunks_per_wave, nchunks);
-    const AllScalars<R> A = load_scalars<R>(B.prm);
 
     // cell of this lane in chunk c: virtual position 58c + lane - 3
     auto locate = [&](unsigned c, long& q, int& i, int& j, int& k, bool& in_range) {
@@ -303,8 +362,6 @@ __global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, i
         if (GRAV) cd[NV - 1] = -B.gforce[3 * q + 0];
     };
 
-    int nclamp = 0, nitmax = 0, nonconv = 0;
-    double vmax = 0.0;
     long qn;
     int in_, jn, kn;
     bool rn;
@@ -373,47 +430,59 @@ __global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, int ghost_mask, i
 
         const bool valid = in_range && lane >= 3 && lane <= 60 && i >= nb && i < nb + Nx;
         if (valid) {
-            count_newton(nit, nitmax, nonconv);
-            nclamp += ncl;
+            count_newton(nit, T.nitmax, T.nonconv);
+            T.nclamp += ncl;
             double pres = cd[VP], rho = cd[VR], u = cd[VU], vv = cd[VV], w = cd[VW];
             const double etot = total_phys<1>(A.rgamma1, pres, rho, u, vv, w);
             M::integrate(A.Is, F, F2, pres, rho, u, vv, w, etot, GRAV ? cd[NV - 1] : 0.0, GRAV);
             double val[5];
             val[0] = pres; val[1] = rho; val[D::U] = u; val[D::V] = vv; val[D::W] = w;
-            store_cell(B.out, q, val, i, j, k, ghost_mask, nx, ny, B.nz, nb);
-            if (do_speed) {
-                const double vs = M::vsound(A.gamma, pres, rho);
-                vmax = tmax(vmax, tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs);
-            }
+            finish_cell<M, EXP>(B, E, A.gamma, q, val, i, j, k, T);
         }
     }
-    flush_counters(B.st, nitmax, nonconv, nclamp, vmax, do_speed != 0);
+}
+
+template <class M, class R, bool GRAV, bool COSMO, bool EXP>
+__global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, Epilogue E, int chunks_per_wave)
+{
+    if (EXP) load_factors(E);
+    const unsigned nchunks = x_chunks(B.nx, B.ny, B.nz, B.nb);
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned c0 = wave * (unsigned)chunks_per_wave;
+    if (c0 >= nchunks) return;                         // whole wave past the end (uniform)
+    const unsigned c1 = min(c0 + (unsigned)chunks_per_wave, nchunks);
+    const AllScalars<R> A = load_scalars<R>(B.prm);
+    Tally T = tally0();
+    x_run<M, R, GRAV, COSMO, EXP>(B, A, E, c0, c1, T);
+    flush_counters(B.st, T, E.do_speed != 0);
 }
 
 // ============================================================================
-// y / z sweeps: one thread streams along a pencil segment.
-template <class M, class R, int DIR, bool GRAV, bool COSMO>
-__global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, int do_speed, int seglen, int nseg)
+// y / z sweeps: one thread streams along segment `seg` (cells
+// [seg*seglen, min((seg+1)*seglen, N))) of pencil `pen` of box B.
+template <int DIR> __device__ __forceinline__ void stream_geometry(const BoxView& B, int& Nax, int& Nbb, long& s_ax,
+                                                                  long& s_b)
+{
+    const int nb = B.nb;
+    if (DIR == 2) { Nax = B.ny - 2 * nb; Nbb = B.nz - 2 * nb; s_ax = B.nx; s_b = (long)B.nx * B.ny; }
+    else { Nax = B.nz - 2 * nb; Nbb = B.ny - 2 * nb; s_ax = (long)B.nx * B.ny; s_b = B.nx; }
+}
+
+template <class M, class R, int DIR, bool GRAV, bool COSMO, bool EXP>
+__device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>& A, const Epilogue& E, long pen,
+                                           int seg, int seglen, bool active, Tally& T)
 {
     using D = Dirmap<DIR>;
     constexpr int NV = GRAV ? 6 : 5;
-    const int nx = B.nx, ny = B.ny, nz = B.nz, nb = B.nb;
+    const int nx = B.nx, nb = B.nb;
     const int Nx = nx - 2 * nb;
     int Nax, Nbb;
     long s_ax, s_b;
-    if (DIR == 2) { Nax = ny - 2 * nb; Nbb = nz - 2 * nb; s_ax = nx; s_b = (long)nx * ny; }
-    else { Nax = nz - 2 * nb; Nbb = ny - 2 * nb; s_ax = (long)nx * ny; s_b = nx; }
-    const long npen = (long)Nx * Nbb;
-    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = tid < npen * nseg;
-    const long tt = active ? tid : 0;
-    const int seg = (int)(tt / npen);
-    const long pen = tt - (long)seg * npen;
+    stream_geometry<DIR>(B, Nax, Nbb, s_ax, s_b);
     const int ia = (int)(pen % Nx), ib = (int)(pen / Nx);
     const int s0 = seg * seglen;
     const int s1 = min(s0 + seglen, Nax);
     const long base = (nb + ia) + (long)(nb + ib) * s_b + (long)nb * s_ax;   // interior cell 0 of the pencil
-    const AllScalars<R> A = load_scalars<R>(B.prm);
     const bool flat_always = A.mingradflat <= 0.0;
 
     double w0[NV], w1[NV], w2[NV], w3[NV];   // cells k-4 .. k-1
@@ -427,8 +496,6 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, i
     for (int s = 0; s < NV; ++s) { w0[s] = w1[s] = w2[s] = w3[s] = iv2[s] = 0.0; d1[s] = d2[s] = R(0); }
     fRprev.p = fRprev.r = fRprev.u = fRprev.v = fRprev.w = 0.0;
     Fprev.p = Fprev.r = Fprev.u = Fprev.v = Fprev.w = 0.0;
-    int nclamp = 0, nitmax = 0, nonconv = 0;
-    double vmax = 0.0;
 
     if (active) {
         double nxt[NV];
@@ -471,14 +538,14 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, i
             if (kk >= s0 + 1) {     // parabola and faces of cell k-2
                 const bool flat = flat_always || bits != 0u || bk;
                 M::template cell_faces<R, NV, GRAV, COSMO>(A, w2, iv2, iv1, w0[VR], w1[VR], w3[VR], cur[VR], d1[VR],
-                                                           d3[VR], w1[VP], w3[VP], flat, fL, fR, nclamp);
+                                                           d3[VR], w1[VP], w3[VP], flat, fL, fR, T.nclamp);
             }
             Flux F = Fprev;
             if (kk >= s0 + 2) {     // Riemann problem at interface k-2
                 int nit;
                 F = M::riemann(A.Rs, fRprev, fL, nit);
                 peak_floor(A.Rs, F, r5, w0[VR], w1[VR], w2[VR], w0[VU], w1[VU]);
-                count_newton(nit, nitmax, nonconv);
+                count_newton(nit, T.nitmax, T.nonconv);
             }
             if (kk >= s0 + 3) {     // update cell k-3
                 double pres = w1[VP], rho = w1[VR], u = w1[VU], vv = w1[VV], w = w1[VW];
@@ -490,11 +557,7 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, i
                 int ix = nb + ia, iy, iz;
                 if (DIR == 2) { iy = ic; iz = nb + ib; }
                 else { iy = nb + ib; iz = ic; }
-                store_cell(B.out, q - 3 * s_ax, val, ix, iy, iz, ghost_mask, nx, ny, nz, nb);
-                if (do_speed) {
-                    const double vs = M::vsound(A.gamma, pres, rho);
-                    vmax = tmax(vmax, tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs);
-                }
+                finish_cell<M, EXP>(B, E, A.gamma, q - 3 * s_ax, val, ix, iy, iz, T);
             }
             // slide the window
             r5 = w0[VR];
@@ -508,13 +571,37 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, int ghost_mask, i
             Fprev = F;
         }
     }
-    flush_counters(B.st, nitmax, nonconv, nclamp, vmax, do_speed != 0);
+}
+
+template <class M, class R, int DIR, bool GRAV, bool COSMO, bool EXP>
+__global__ void __launch_bounds__(256) sweep_stream(BoxView B, Epilogue E, int seglen, int nseg)
+{
+    if (EXP) load_factors(E);
+    int Nax, Nbb;
+    long s_ax, s_b;
+    stream_geometry<DIR>(B, Nax, Nbb, s_ax, s_b);
+    const long npen = (long)(B.nx - 2 * B.nb) * Nbb;
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = tid < npen * nseg;
+    const long tt = active ? tid : 0;
+    const int seg = (int)(tt / npen);
+    const long pen = tt - (long)seg * npen;
+    const AllScalars<R> A = load_scalars<R>(B.prm);
+    Tally T = tally0();
+    stream_run<M, R, DIR, GRAV, COSMO, EXP>(B, A, E, pen, seg, seglen, active, T);
+    flush_counters(B.st, T, E.do_speed != 0);
 }
 
 // host launcher of one sweep
-template <class M, class R, bool G, bool C>
+template <class M, class R, bool G, bool C, bool X>
 static hipError_t launch_sweep_t(const BoxView& b, const SweepLaunch& L, hipStream_t s)
 {
+    Epilogue E;
+    for (int f = 0; f < 4; ++f) E.old[f] = L.old[f];
+    E.fac = L.fac;
+    E.afact = E.afact1 = E.afact2 = 0.0;
+    E.ghost_mask = L.ghost_mask;
+    E.do_speed = L.do_speed;
     const int nb = b.nb;
     const int Nx = b.nx - 2 * nb, Ny = b.ny - 2 * nb, Nz = b.nz - 2 * nb;
     const int block = L.block > 0 ? L.block : 256;
@@ -529,8 +616,7 @@ static hipError_t launch_sweep_t(const BoxView& b, const SweepLaunch& L, hipStre
         const long waves = (nchunks + cpw - 1) / cpw;
         const long threads = waves * 64;
         const long grid = (threads + block - 1) / block;
-        hipLaunchKernelGGL((sweep_lane_x<M, R, G, C>), dim3((unsigned)grid), dim3(block), 0, s, b, L.ghost_mask,
-                           L.do_speed, (int)cpw);
+        hipLaunchKernelGGL((sweep_lane_x<M, R, G, C, X>), dim3((unsigned)grid), dim3(block), 0, s, b, E, (int)cpw);
     } else {
         const int Nax = (L.dir == 2) ? Ny : Nz;
         const int Nbb = (L.dir == 2) ? Nz : Ny;
@@ -540,13 +626,27 @@ static hipError_t launch_sweep_t(const BoxView& b, const SweepLaunch& L, hipStre
         const long threads = (long)Nx * Nbb * nseg;
         const long grid = (threads + block - 1) / block;
         if (L.dir == 2)
-            hipLaunchKernelGGL((sweep_stream<M, R, 2, G, C>), dim3((unsigned)grid), dim3(block), 0, s, b,
-                               L.ghost_mask, L.do_speed, seglen, nseg);
+            hipLaunchKernelGGL((sweep_stream<M, R, 2, G, C, X>), dim3((unsigned)grid), dim3(block), 0, s, b, E,
+                               seglen, nseg);
         else
-            hipLaunchKernelGGL((sweep_stream<M, R, 3, G, C>), dim3((unsigned)grid), dim3(block), 0, s, b,
-                               L.ghost_mask, L.do_speed, seglen, nseg);
+            hipLaunchKernelGGL((sweep_stream<M, R, 3, G, C, X>), dim3((unsigned)grid), dim3(block), 0, s, b, E,
+                               seglen, nseg);
     }
     return hipGetLastError();
+}
+
+// dispatch over gravity / comoving / fused expansion (expansion only with comoving)
+template <class M, class R>
+static hipError_t launch_sweep_variants(const BoxView& b, const SweepLaunch& L, hipStream_t s)
+{
+    const bool x = L.old[0] != nullptr;
+    if (x && !L.cosmo) return hipErrorInvalidValue;
+    if (L.grav) {
+        if (!L.cosmo) return launch_sweep_t<M, R, true, false, false>(b, L, s);
+        return x ? launch_sweep_t<M, R, true, true, true>(b, L, s) : launch_sweep_t<M, R, true, true, false>(b, L, s);
+    }
+    if (!L.cosmo) return launch_sweep_t<M, R, false, false, false>(b, L, s);
+    return x ? launch_sweep_t<M, R, false, true, true>(b, L, s) : launch_sweep_t<M, R, false, true, false>(b, L, s);
 }
 
 }  // namespace hp

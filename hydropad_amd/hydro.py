@@ -150,6 +150,25 @@ class Box:
         self.params.rdtath = dd.value
         return tt.value, dd.value
 
+    # ---- comoving boxes ----
+    def set_cosmo(self, sched):
+        """Take the schedule's at, dat, rat, dt, rdtath as the sweep scalars."""
+        check("hp_cosmo_params", self.L.hp_cosmo_params(ctypes.byref(sched), ctypes.byref(self.params)))
+        self.set_params(self.params)
+
+    def savetn(self):
+        check("hp_savetn", self.L.hp_savetn(self._h))
+
+    def expand(self, afact):
+        check("hp_expand", self.L.hp_expand(self._h, afact))
+
+    def evolve_cosmo(self, first_step, nsteps, sched):
+        """Device-resident comoving time loop; advances sched (HpCosmo) in place."""
+        check("hp_evolve_cosmo", self.L.hp_evolve_cosmo(self._h, int(first_step), int(nsteps),
+                                                        ctypes.byref(sched)))
+        check("hp_cosmo_params", self.L.hp_cosmo_params(ctypes.byref(sched), ctypes.byref(self.params)))
+        return sched
+
     def sweep_timing_enable(self, on=True):
         check("hp_sweep_timing_enable", self.L.hp_sweep_timing_enable(self._h, 1 if on else 0))
 

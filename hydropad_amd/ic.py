@@ -129,3 +129,81 @@ def sod_exact(x, t, gamma=5.0 / 3.0, left=(1.0, 1.0), right=(0.125, 0.1), x0=0.5
     rho[regions[3]], u[regions[3]], p[regions[3]] = rsr, us, ps
     rho[regions[4]], u[regions[4]], p[regions[4]] = rr, 0.0, pr
     return rho, u, p
+
+
+# ---- comoving boxes ---------------------------------------------------------
+# FP32 exponents of the Einstein-de Sitter schedule (hydropad/functions.F90:67-68)
+_K23 = float(np.float32(2.0) / np.float32(3.0))
+_KM13 = float(np.float32(-1.0) / np.float32(3.0))
+
+
+def eds_adot(a_start):
+    """da/dt at the start of the first step of a box started at a_start, as the
+    schedule computes it (initialize_parameters.F90:138-151: t = a^1.5,
+    dat = (2/3) t^(-1/3) with the FP32 exponents)."""
+    return _K23 * (a_start ** 1.5) ** _KM13
+
+
+def eds_schedule(a_start, a_end, nsteps):
+    """(ca, tstop) such that the Einstein-de Sitter schedule started at a_start
+    ends at a = a_end after nsteps steps when the da/a <= ca limit sets dt
+    (hydropad/functions.F90:69,87-88): the first step is dt = 0.010 tin
+    (initialize_parameters.F90:141), every later one grows a by (1+ca), and the
+    last step is clipped at tstop = a_end^(1/K), K the FP32 2/3."""
+    tin = a_start ** 1.5
+    a1 = (tin * (1.0 + float(np.float32(0.010)))) ** _K23
+    ca = (a_end / a1) ** (1.0 / (nsteps - 1)) - 1.0
+    return ca * (1.0 + 1e-4), a_end ** (1.0 / _K23)
+
+
+def grf_box(N, seed, slope=-2.0, rms=0.05, mach=50.0, vfac=None, a_start=1.0 / 51.0, gamma=5.0 / 3.0):
+    """Seeded Gaussian random density field with Zel'dovich velocities at
+    uniform temperature, on the (N+6)^3 periodic layout.
+
+      delta    P(k) ~ k^slope for 2 pi <= |k| <= pi N (box length 1; the
+               single-axis Nyquist planes left empty), zero mean, scaled to
+               the given rms; rho = 1 + delta
+      v        Zel'dovich growing mode: displacement psi_k = i k delta_k / k^2
+               (div psi = -delta) times vfac, by default da/dt at a_start
+               (peculiar velocity of linear theory, f = 1 in EdS)
+      p        uniform temperature: p = rho cs^2 / gamma with
+               cs = v_rms / mach
+    The generator is numpy's PCG64 seeded with `seed`; the reference has no
+    generator of its own (its initial conditions are read from files,
+    hydropad/initialdm.F90:37-70)."""
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((N, N, N))
+    wk = np.fft.rfftn(white)
+    kx = 2.0 * np.pi * np.fft.fftfreq(N, d=1.0 / N)
+    kz = 2.0 * np.pi * np.fft.rfftfreq(N, d=1.0 / N)
+    KX, KY, KZ = np.meshgrid(kx, kx, kz, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    k = np.sqrt(k2)
+    # the single-axis Nyquist planes carry no derivative (i k X is not
+    # Hermitian there), so they are left empty along with |k| beyond pi N
+    kmax = np.pi * N * (1.0 - 1e-12)
+    band = ((k >= 2.0 * np.pi * (1.0 - 1e-12)) & (k <= np.pi * N * (1.0 + 1e-12))
+            & (np.abs(KX) < kmax) & (np.abs(KY) < kmax) & (np.abs(KZ) < kmax))
+    amp = np.zeros_like(k)
+    amp[band] = k[band] ** (0.5 * slope)
+    dk = wk * amp
+    delta = np.fft.irfftn(dk, s=(N, N, N), axes=(0, 1, 2))
+    f = rms / delta.std()
+    dk *= f
+    delta *= f
+    k2s = np.where(k2 > 0.0, k2, 1.0)
+    if vfac is None:
+        vfac = eds_adot(a_start)
+    vel = [vfac * np.fft.irfftn(1j * K * dk / k2s, s=(N, N, N), axes=(0, 1, 2)) for K in (KX, KY, KZ)]
+    rho = 1.0 + delta
+    if rho.min() <= 0.0:
+        raise ValueError("rms too large: negative density")
+    vrms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
+    cs = vrms / mach
+    n = N + 2 * NB
+    st = empty_state((n, n, n))
+    sl = tuple(slice(NB, NB + N) for _ in range(3))
+    st["rho"][sl] = rho
+    st["p"][sl] = rho * cs * cs / gamma
+    st["vx"][sl], st["vy"][sl], st["vz"][sl] = vel
+    return periodic_fill(st)

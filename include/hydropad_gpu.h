@@ -123,6 +123,48 @@ int hp_speed(hp_ctx* ctx, double* velmax);
 int hp_evolve(hp_ctx* ctx, int first_step, int nsteps, double cour, double tstop,
               double* t, double* dt);
 
+/* ---- comoving (cosmological) boxes ----------------------------------------
+ * Einstein-de Sitter schedule state, the members of hydropad/scalar.F90:16-19
+ * that timedet_cosmo reads and writes.  at/dat are a and da/dt at the start
+ * of the upcoming step, ath/dath at its midpoint, atnew/datnew at its end,
+ * t its end time and dt its length; nstep counts completed steps. */
+typedef struct hp_cosmo {
+    double cour, ca, tstop;      /* Courant number, max da/a per step, stop time */
+    double t, dt;
+    double at, dat, ath, dath, atnew, datnew;
+    double rat, rdtath;          /* 1/at, dt/ath */
+    int64_t nstep;
+} hp_cosmo;
+
+/* Initial schedule for a box starting at expansion factor a_start; replaces
+ * the COSMO / omega_m = 1 branch of hydropad/initialize_parameters.F90:136-178
+ * (tin = a^1.5, first dt = 0.010*tin).  Host-only. */
+int hp_cosmo_init(double a_start, double cour, double ca, double tstop, hp_cosmo* s);
+
+/* SUBROUTINE timedet_cosmo, omega_m = 1 branch (hydropad/functions.F90:38-151),
+ * after step s->nstep+1 with that step's CFL maximum velmax; dx is the cell
+ * size.  The reference's dt0 = 1e-9 cap (functions.F90:49,97) is not applied
+ * and there are no particle speeds.  Host-only. */
+int hp_timedet_cosmo(hp_cosmo* s, double velmax, double dx);
+
+/* Copy the schedule's at, dat, rat, dt, rdtath into the sweep scalars. */
+int hp_cosmo_params(const hp_cosmo* s, hp_params* prm);
+
+/* Keep p, vx, vy, vz of the current state as the start-of-step state the
+ * expansion step uses (the reference's vxold/vyold/vzold/pold3d, hydropad/matrix.F90). */
+int hp_savetn(hp_ctx* ctx);
+
+/* SUBROUTINE expand (hydropad/expand.F90:1-43) on the device, over the whole
+ * array, with afact = dt*(dath/ath) from the caller. */
+int hp_expand(hp_ctx* ctx, double afact);
+
+/* Device-resident cosmological time loop: for each step, ppm_lxlylz in the
+ * nrot order with the schedule's scalars, expand (fused into the step's last
+ * sweep), speed and timedet_cosmo (on the device) -- hydropad/evolve_sys.F90
+ * with COSMO defined.  The schedule *s is read at entry and written back at
+ * return. */
+int hp_evolve_cosmo(hp_ctx* ctx, int first_step, int nsteps, hp_cosmo* s);
+
 /* Per-sweep kernel timing with HIP events on the context's stream; replaces
  * the MPI_WTIME "Elapsed time" print around every ppmsolver call
  * (hydro/ppmsolver.F90:72,464-465).  While enabled, every sweep launch is

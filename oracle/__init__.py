@@ -44,6 +44,19 @@ class HpCounters(ctypes.Structure):
                 ("neg_state_clamps", ctypes.c_int64), ("reserved", ctypes.c_int64 * 5)]
 
 
+COSMO_NAMES = ("cour", "ca", "tstop", "t", "dt", "at", "dat", "ath", "dath", "atnew", "datnew", "rat", "rdtath")
+
+
+class HpCosmo(ctypes.Structure):
+    """Mirror of hp_cosmo (the Einstein-de Sitter schedule state)."""
+    _fields_ = [(n, _D) for n in COSMO_NAMES] + [("nstep", ctypes.c_int64)]
+
+    def copy(self):
+        c = HpCosmo()
+        ctypes.pointer(c)[0] = self
+        return c
+
+
 def build(ref=True):
     """Compile liboracle.so (and the reference build when its tree exists)."""
     target = "all" if ref else "oracle"
@@ -90,6 +103,15 @@ class Oracle:
         L.or_evolve.argtypes = [PP] + [_P] * 6 + [_I] * 6 + [_D, _D, ctypes.POINTER(_D),
                                                               ctypes.POINTER(HpCounters)]
         L.or_evolve.restype = _I
+        PC = ctypes.POINTER(HpCosmo)
+        L.or_cosmo_init.argtypes = [_D, _D, _D, _D, PC]
+        L.or_cosmo_init.restype = None
+        L.or_timedet_cosmo.argtypes = [PC, _D, _D]
+        L.or_timedet_cosmo.restype = None
+        L.or_expand.argtypes = [_D] + [_P] * 8 + [ctypes.c_long]
+        L.or_expand.restype = None
+        L.or_evolve_cosmo.argtypes = [PP] + [_P] * 6 + [_I] * 6 + [PC, ctypes.POINTER(HpCounters)]
+        L.or_evolve_cosmo.restype = _I
         L.or_set_threads.argtypes = [_I]
         L.or_nrot_dir.argtypes = [_I, _I]
         L.or_nrot_dir.restype = _I
@@ -185,6 +207,30 @@ class Oracle:
 
     def nrot_dir(self, nrot, s):
         return self.L.or_nrot_dir(nrot, s)
+
+    # ---- comoving boxes ----
+    def cosmo_init(self, a_start, cour, ca, tstop):
+        s = HpCosmo()
+        self.L.or_cosmo_init(a_start, cour, ca, tstop, ctypes.byref(s))
+        return s
+
+    def timedet_cosmo(self, s, velmax, dx):
+        self.L.or_timedet_cosmo(ctypes.byref(s), velmax, dx)
+        return s
+
+    def expand(self, afact, state, old):
+        f = self._fields(state)
+        o = [np.asfortranarray(old[k], dtype=np.float64) for k in ("p", "vx", "vy", "vz")]
+        self.L.or_expand(afact, _ptr(f[0]), _ptr(f[2]), _ptr(f[3]), _ptr(f[4]), *[_ptr(a) for a in o], f[0].size)
+
+    def evolve_cosmo(self, P, state, first_step, nsteps, sched, nbound=3, gforce=None, counters=None):
+        """Comoving time loop; sched (HpCosmo) is advanced in place, P takes its scalars."""
+        f = self._fields(state)
+        nx, ny, nz = f[0].shape
+        cnt = counters if counters is not None else HpCounters()
+        g = None if gforce is None else np.asfortranarray(gforce, dtype=np.float64)
+        return self.L.or_evolve_cosmo(ctypes.byref(P), *[_ptr(a) for a in f], _ptr(g), nx, ny, nz, nbound,
+                                      first_step, nsteps, ctypes.byref(sched), ctypes.byref(cnt))
 
 
 class Reference:

@@ -732,3 +732,109 @@ int or_evolve(hp_params* P, double* p, double* rho, double* vx, double* vy, doub
     }
     return worst;
 }
+
+/* ---- comoving boxes (COSMO, Einstein-de Sitter) ---------------------------
+ * FP32 exponents of the reference, widened: 2.0/3.0 and -1.0/3.0 are default
+ * REAL quotients (hydropad/functions.F90:67-68,145-148); 0.010 likewise
+ * (hydropad/initialize_parameters.F90:141). */
+#define F_TWO_THIRDS   ((double)(2.0f / 3.0f))
+#define F_MINUS_THIRD  ((double)(-1.0f / 3.0f))
+
+/* hydropad/initialize_parameters.F90:136-178 (COSMO, omega_m = 1, no restart),
+ * with at = a_start as set by the initial-conditions reader (initialdm.F90:55). */
+void or_cosmo_init(double a_start, double cour, double ca, double tstop, hp_cosmo* s)
+{
+    double at = a_start;
+    double tin = pow(at, 1.5);
+    double told = tin;
+    double dt = F_0010 * tin;
+    double t = tin + dt;
+    double thalf = told + dt / 2.0;
+    s->cour = cour; s->ca = ca; s->tstop = tstop;
+    s->at = pow(told, F_TWO_THIRDS);
+    s->atnew = pow(t, F_TWO_THIRDS);
+    s->dat = F_TWO_THIRDS * pow(told, F_MINUS_THIRD);
+    s->ath = pow(told + dt / 2., F_TWO_THIRDS);
+    s->dath = F_TWO_THIRDS * pow(thalf, F_MINUS_THIRD);
+    s->datnew = F_TWO_THIRDS * pow(t, F_MINUS_THIRD);
+    s->rat = 1.0 / s->at;
+    s->rdtath = dt / s->ath;
+    s->t = t; s->dt = dt;
+    s->nstep = 0;
+}
+
+/* SUBROUTINE timedet_cosmo, omega_m = 1 (hydropad/functions.F90:38-151), run
+ * at the end of step s->nstep+1.  Not applied: the dt0 = 1e-9 cap (:49,97,
+ * which would make any run take ~1e7 steps) and the particle speed (:75-84,
+ * uninitialised in the reference; no particles here). */
+void or_timedet_cosmo(hp_cosmo* s, double velmax, double dx)
+{
+    s->nstep += 1;
+    double told = s->t;
+    double rdt = (s->nstep > 1) ? 5.0 / (6.0 * s->dt) : 0.0;
+    s->at = pow(s->t, F_TWO_THIRDS);
+    s->dat = F_TWO_THIRDS * pow(s->t, F_MINUS_THIRD);
+    double dta = pow((1.0 + s->ca) * s->at, 1.5) - s->t;
+    s->rat = 1.0 / s->at;
+    double rdtc = velmax / (s->cour * dx * s->at);
+    double dt = 1.0 / dmax2(rdt, rdtc);
+    dt = dmin(dt, dta);
+    if (dt >= 10.0) dt = 0.1;
+    double t = s->t + dt;
+    if (t > s->tstop) { t = s->tstop; dt = t - told; }
+    double dth = 0.5 * dt;
+    double thalf = told + dth;
+    s->t = t; s->dt = dt;
+    s->atnew = pow(t, F_TWO_THIRDS);
+    s->ath = pow(thalf, F_TWO_THIRDS);
+    s->dath = F_TWO_THIRDS * pow(thalf, F_MINUS_THIRD);
+    s->datnew = F_TWO_THIRDS * pow(t, F_MINUS_THIRD);
+    s->rdtath = dt / s->ath;
+}
+
+/* SUBROUTINE expand (hydropad/expand.F90:1-43) over n cells, in place. */
+void or_expand(double afact, double* p, double* vx, double* vy, double* vz, const double* pold,
+               const double* vxold, const double* vyold, const double* vzold, long n)
+{
+    double afact1 = 1.0 / (1.0 + afact);
+    double afact2 = 1.0 / (2.0 + afact);
+    for (long i = 0; i < n; ++i) {
+        vx[i] = (2.0 * vx[i] - afact * vxold[i]) * afact2;
+        vy[i] = (2.0 * vy[i] - afact * vyold[i]) * afact2;
+        vz[i] = (2.0 * vz[i] - afact * vzold[i]) * afact2;
+        double ppp = p[i];
+        p[i] = (p[i] - afact * pold[i]) * afact1;
+        if (p[i] <= 0.0) p[i] = (ppp > 0.0) ? ppp : pold[i];
+    }
+}
+
+/* evolve_sys with COSMO (hydropad/evolve_sys.F90:30-31,54,94-96,122-127): per
+ * step keep p, vx, vy, vz, ppm_lxlylz with the schedule's scalars, expand with
+ * afact = dt*(dath/ath), speed, timedet_cosmo. */
+int or_evolve_cosmo(hp_params* P, double* p, double* rho, double* vx, double* vy, double* vz,
+                    const double* gforce, int nx, int ny, int nz, int nb, int first_step, int nsteps,
+                    hp_cosmo* s, hp_counters* cnt)
+{
+    int worst = 0;
+    long n = (long)nx * ny * nz;
+    double* old = (double*)malloc(4 * n * sizeof(double));
+    if (!old) return HP_E_NOMEM;
+    for (int nstep = first_step; nstep < first_step + nsteps; ++nstep) {
+        int nrot = nstep % 6 + 1;
+        if (nstep == 1) nrot = 1;
+        P->at = s->at; P->dat = s->dat; P->rat = s->rat; P->dt = s->dt; P->rdtath = s->rdtath;
+        memcpy(old, p, n * sizeof(double));
+        memcpy(old + n, vx, n * sizeof(double));
+        memcpy(old + 2 * n, vy, n * sizeof(double));
+        memcpy(old + 3 * n, vz, n * sizeof(double));
+        int r = or_ppm_lxlylz(P, p, rho, vx, vy, vz, gforce, nx, ny, nz, nb, nrot, cnt);
+        if (r < 0) worst = r;
+        double afact = s->dt * (s->dath / s->ath);
+        or_expand(afact, p, vx, vy, vz, old, old + n, old + 2 * n, old + 3 * n, n);
+        double vmax = or_speed(P, p, rho, vx, vy, vz, nx, ny, nz, nb);
+        or_timedet_cosmo(s, vmax, P->dx);
+    }
+    P->at = s->at; P->dat = s->dat; P->rat = s->rat; P->dt = s->dt; P->rdtath = s->rdtath;
+    free(old);
+    return worst;
+}
