@@ -69,7 +69,11 @@ EXPORTS = ("hp_params_init", "hp_create", "hp_destroy", "hp_set_precision", "hp_
            "hp_ppmsolver", "hp_exchange_mesh", "hp_ppm_lxlylz", "hp_speed", "hp_evolve",
            "hp_status", "hp_reset_counters", "hp_sync", "hp_integrate_cell", "hp_dualenergy_cell",
            "hp_sweep_timing_enable", "hp_sweep_timing", "hp_cosmo_init", "hp_timedet_cosmo",
-           "hp_cosmo_params", "hp_savetn", "hp_expand", "hp_evolve_cosmo")
+           "hp_cosmo_params", "hp_savetn", "hp_expand", "hp_evolve_cosmo",
+           "hp_batch_box_bytes", "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_add_sod",
+           "hp_sod_of_seed", "hp_batch_remove", "hp_batch_download", "hp_batch_step", "hp_batch_sync",
+           "hp_batch_poll", "hp_batch_use_graph", "hp_batch_set_tuning", "hp_batch_get_tuning",
+           "hp_batch_bytes_used", "hp_batch_autotune", "hp_serve_run")
 
 _lib = None
 

@@ -19,7 +19,8 @@ ARCH = os.environ.get("HP_OFFLOAD_ARCH", "gfx950")
 # (source, FP contraction): the exact sweep keeps the reference's operation
 # order, so it is built without contraction; the throughput sweeps with it.
 SOURCES = [("hp_kernels.hip", "off"), ("hp_kernels_exact.hip", "off"), ("hp_kernels_fast.hip", "fast"),
-           ("hp_capi.hip", "off")]
+           ("hp_capi.hip", "off"), ("hp_batch_kernels.hip", "fast"), ("hp_batch.hip", "off"),
+           ("hp_serve.hip", "off")]
 
 
 def hipcc():

@@ -29,6 +29,12 @@ struct DevStatus {
     double t;                          // time at the end of the upcoming step (timedet semantics)
     double fac[3];                     // expansion factors of the upcoming step (expand.F90:18-20)
     Cosmo cosmo;                       // schedule of a cosmological box
+    // run control of a box in a batch (unused by a single context)
+    long long nstep_done;              // steps completed since the box was added
+    long long nstep_target;            // steps to run
+    int done;                          // nstep_done reached nstep_target
+    int comoving;
+    double cour, tstop;                // non-comoving timedet inputs
 };
 
 // One box as seen by a sweep.  Fields are (nx,ny,nz) arrays with pitch nx,

@@ -22,6 +22,7 @@
 #ifndef HYDROPAD_GPU_H
 #define HYDROPAD_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -178,6 +179,119 @@ int hp_sweep_timing(hp_ctx* ctx, double ms[3], int64_t launches[3]);
 int hp_status(hp_ctx* ctx, hp_counters* out);
 int hp_reset_counters(hp_ctx* ctx);
 int hp_sync(hp_ctx* ctx);
+
+/* ---- batched ensemble ----------------------------------------------------------
+ * Many independent periodic boxes, each with its own size, precision, step
+ * count, time step and (for comoving boxes) expansion schedule, resident on
+ * one GPU and advanced together.  Each batched step runs, for every box, one
+ * step of the evolve_sys loop above (its own nrot order, hydropad/evolve_sys.F90:30-31;
+ * its own timedet or timedet_cosmo): a plan kernel, three sweep slots of
+ * persistent per-direction kernels, one timedet kernel -- no host work and
+ * no synchronisation, so the sequence is captured in a hipGraph when asked.
+ * The reference has no counterpart (it advances one box per MPI job); the
+ * per-box arithmetic is that of HP_FP64 / HP_MIXED, and the tiling does not
+ * change any result.  Gravity is not supported in a batch. */
+typedef struct hp_batch hp_batch;
+
+typedef struct hp_batch_tuning {
+    int x_chunks;         /* x sweep: 58-cell chunks per work unit (0: 4) */
+    int yz_segments;      /* y/z sweeps: segments each pencil is split into (0: 1) */
+    int waves_per_cu;     /* persistent grid: waves per compute unit (0: 8) */
+    int reserved;
+} hp_batch_tuning;
+
+typedef struct hp_box_status {
+    int64_t steps_done;
+    int32_t active, done;
+    double t, dt;         /* end time of the step in flight / next step, its dt */
+    int64_t newton_iters_max, newton_nonconv, neg_state_clamps;
+} hp_box_status;
+
+/* Device bytes of one box's field sets ((nx,ny,nz) with ghosts; comoving
+ * boxes keep a third set for the expansion step). */
+size_t hp_batch_box_bytes(int nx, int ny, int nz, int comoving);
+
+int  hp_batch_create(int max_boxes, size_t hbm_budget_bytes, hp_batch** out);
+void hp_batch_destroy(hp_batch* b);
+
+/* Add a box from host arrays (Fortran order, ghosts included) to run nsteps
+ * steps.  cosmo == NULL: static box, first dt from its own CFL condition
+ * (speed + timedet with t = 0, cour, tstop); otherwise a comoving box on that
+ * schedule.  Returns HP_E_NOMEM when the slot table or the HBM budget is full. */
+int hp_batch_add(hp_batch* b, int nx, int ny, int nz, int nbound, int precision, double gamma,
+                 double cour, double tstop, const hp_cosmo* cosmo, int nsteps,
+                 const double* p, const double* rho, const double* vx, const double* vy,
+                 const double* vz, int* id);
+
+/* Add an n^3 static box holding the seeded planar Sod tube, generated on the
+ * device (gamma = 5/3, no upload); see hp_sod_of_seed. */
+int hp_batch_add_sod(hp_batch* b, int n, int precision, uint64_t seed, double cour, int nsteps, int* id);
+
+/* What a seed selects for the Sod tube of hp_batch_add_sod: the axis (0..2),
+ * the split position x0 in [0.25, 0.75) and whether the high-pressure side is
+ * flipped to x >= x0 (from splitmix64 of the seed).  Host-only. */
+int hp_sod_of_seed(uint64_t seed, int* axis, double* x0, int* flip);
+
+int hp_batch_remove(hp_batch* b, int id);
+int hp_batch_download(hp_batch* b, int id, double* p, double* rho, double* vx, double* vy, double* vz);
+
+/* Enqueue nsteps batched steps (asynchronous); boxes that reached their step
+ * count are left alone. */
+int hp_batch_step(hp_batch* b, int nsteps);
+int hp_batch_sync(hp_batch* b);
+
+/* Synchronise and report every slot (array of max_boxes entries). */
+int hp_batch_poll(hp_batch* b, hp_box_status* out, int* nactive);
+
+/* Capture each batched step in a hipGraph and replay it (re-captured when the
+ * mix of precisions / comoving boxes or the grid changes). */
+int hp_batch_use_graph(hp_batch* b, int on);
+
+int hp_batch_set_tuning(hp_batch* b, const hp_batch_tuning* t);
+int hp_batch_get_tuning(hp_batch* b, hp_batch_tuning* t);
+size_t hp_batch_bytes_used(const hp_batch* b);
+
+/* Autotuner: coordinate search over x_chunks, yz_segments and waves_per_cu,
+ * each trial steps_per_trial real batched steps timed with HIP events; leaves
+ * the fastest tiling set.  *steps_run receives the steps it advanced. */
+int hp_batch_autotune(hp_batch* b, int steps_per_trial, hp_batch_tuning* best,
+                      double* best_ms_per_step, int* steps_run);
+
+/* ---- serving ---------------------------------------------------------------
+ * A request loop with continuous batching: requests (a box size, a step
+ * count, a precision and a seed for the Sod initial state) arrive at
+ * arrival_s seconds after the call starts, are admitted in arrival order
+ * while box slots and the HBM budget allow, advance one batched step at a
+ * time together with everything resident, and retire when their step count
+ * is reached.  result_fields, if not NULL, holds 5 host pointers per request
+ * (p, rho, vx, vy, vz of (n+6)^3 doubles, or NULL) for the final states. */
+typedef struct hp_request {
+    int64_t id;
+    int32_t n, nsteps;
+    int32_t precision, pad;
+    uint64_t seed;
+    double arrival_s;
+    double cour;
+} hp_request;
+
+typedef struct hp_request_result {
+    int64_t id;
+    double arrival_s, admit_s, done_s;
+    int64_t steps;
+    double t;
+    int64_t newton_nonconv;
+    int32_t rc, pad;
+} hp_request_result;
+
+typedef struct hp_serve_stats {
+    double wall_s, cell_updates;
+    double p50_s, p99_s, mean_s;
+    int64_t batched_steps, max_concurrent, completed;
+    double peak_bytes;
+} hp_serve_stats;
+
+int hp_serve_run(const hp_request* reqs, int nreq, size_t hbm_budget, int max_boxes, int use_graph,
+                 double* const* result_fields, hp_request_result* results, hp_serve_stats* stats);
 
 /* ---- scalar entry points (host-callable, for unit vectors) ---------------- */
 /* SUBROUTINE integrate (hydro/integrate.F90:1-113) on one cell, arguments by

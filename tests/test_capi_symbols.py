@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "hydropad_gpu.h")
 def declared_symbols():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void)\s+(hp_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|size_t)\s+(hp_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_declares_entry_points():
@@ -61,3 +61,30 @@ def test_bad_arguments_rejected_without_gpu():
     assert L.hp_create(70, 70, 70, 2, 0, ctypes.byref(h)) == _capi.HP_E_ARG     # nbound must be 3
     assert L.hp_create(70, 70, 70, 3, 7, ctypes.byref(h)) == _capi.HP_E_ARG     # unknown precision
     assert L.hp_ppmsolver(None, 1) == _capi.HP_E_ARG
+
+
+def test_batch_host_helpers_without_gpu():
+    """Host-only pieces of the batch API: memory accounting, the seed -> Sod
+    mapping, and argument checks that return before any HIP call."""
+    from hydropad_amd import _capi, batch
+    if not os.path.exists(_capi.LIBPATH):
+        pytest.skip("libhydropad_amd.so not built")
+    fb = -(-70 ** 3 * 8 // 256) * 256
+    assert batch.box_bytes((70, 70, 70)) == 10 * fb
+    assert batch.box_bytes((70, 70, 70), comoving=True) == 15 * fb
+    seen_axis, seen_flip, xs = set(), set(), []
+    for seed in range(300):
+        a, x0, f = batch.sod_of_seed(seed)
+        assert a in (0, 1, 2) and f in (0, 1) and 0.25 <= x0 < 0.75
+        seen_axis.add(a)
+        seen_flip.add(f)
+        xs.append(x0)
+    assert seen_axis == {0, 1, 2} and seen_flip == {0, 1}
+    assert np.std(xs) > 0.1
+    assert batch.sod_of_seed(12345) == batch.sod_of_seed(12345)
+    L = batch._lib()
+    h = ctypes.c_void_p()
+    assert L.hp_batch_create(0, 1 << 30, ctypes.byref(h)) == _capi.HP_E_ARG
+    assert L.hp_batch_create(100000, 1 << 30, ctypes.byref(h)) == _capi.HP_E_ARG
+    assert L.hp_batch_step(None, 1) == _capi.HP_E_ARG
+    assert L.hp_serve_run(None, 0, 0, 1, 0, None, None, None) == _capi.HP_E_ARG
