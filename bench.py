@@ -9,12 +9,20 @@ from the same initial state; the metric is cell-updates/s (N^3 * K / wall
 seconds) with the state resident in HBM, and the mixed form's per-field
 relative L1/Linf against the FP64 run.
 
-    python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--size 512]
+    python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--size 512] [--no-extras]
 
 Prints one JSON line.  `roofline` is the dominant sweep kernel's achieved HBM
 bandwidth (80 algorithmic bytes per cell per sweep / its average HIP-event
-duration) against 8 TB/s; `cpu_baseline` times the reference's own Fortran
-sweep (oracle/_ref, one core) on a bounded slab of the same workload.
+duration) against 8 TB/s; `cpu_baseline` times the oracle's C restatement of
+the reference sweep (bit-identical to the reference's Fortran, one core) on a
+bounded slab of the same workload, with the reference itself timed beside it
+where its build is present.  Beside the headline the line carries the other
+workloads of the metric:
+  cosmo_256   comoving 256^3 box, a = 1/51 -> 1/11 in 200 steps, FP64 (and mixed)
+  ensemble    64 comoving boxes of sizes {32,48,64,96,128,192}^3, own seeds,
+              P(k) slopes and rms, one batch, autotuned, hipGraph; FP64 and mixed
+  serving     Poisson stream of Sod-tube requests (N log-uniform 32..256,
+              20..200 steps, FP64 or mixed), continuous batching; p50/p99 latency
 """
 import argparse
 import json
@@ -164,6 +172,112 @@ def cpu_baseline(N, budget_s=12.0):
     return out
 
 
+def run_cosmo(hp, ic, N=256, steps=200):
+    """Comoving box of the cosmological workload: cell-updates/s per precision,
+    mixed-vs-FP64 error."""
+    st = ic.grf_box(N, 2026)
+    ca, tstop = ic.eds_schedule(1.0 / 51.0, 1.0 / 11.0, steps)
+    out = {"workload": f"grf_{N}^3_eds_a1/51-1/11_{steps}steps_mach50", "box": N, "steps": steps}
+    states = {}
+    for prec in ("fp64", "mixed"):
+        with hp.Box(N, prec) as box:
+            s = hp.cosmo_init(1.0 / 51.0, 0.8, ca, tstop)
+            box.upload(st)
+            box.sync()
+            t0 = time.perf_counter()
+            box.evolve_cosmo(1, steps, s)
+            wall = time.perf_counter() - t0
+            states[prec] = box.download()
+        out[f"{prec}_cell_updates_per_s"] = round(N ** 3 * steps / wall, 1)
+        out[f"{prec}_ms_per_step"] = round(wall / steps * 1e3, 4)
+        out["a_end"] = s.atnew
+    out["mixed_vs_fp64_err"] = errors(states["mixed"], states["fp64"])
+    return out
+
+
+def ensemble_boxes(ic, nbox=64, seed=4242):
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([32, 48, 64, 96, 128, 192], size=nbox)
+    boxes = []
+    for i, n in enumerate(sizes):
+        slope = float(rng.uniform(-2.5, -1.5))
+        rms = float(rng.uniform(0.02, 0.1))
+        boxes.append((int(n), 1000 + i, slope, rms))
+    return boxes
+
+
+def run_ensemble(hp, ic, steps=50, nbox=64):
+    """64-box comoving ensemble in one batch (persistent per-direction kernels,
+    hipGraph, autotuned tiling), FP64 and mixed, each from the same states."""
+    from hydropad_amd import batch
+    spec = ensemble_boxes(ic, nbox)
+    log(f"bench: ensemble ICs ({nbox} boxes)")
+    states = [ic.grf_box(n, sd, slope=sl, rms=rms) for n, sd, sl, rms in spec]
+    cells = sum(n ** 3 for n, _, _, _ in spec)
+    ca, tstop = ic.eds_schedule(1.0 / 51.0, 1.0 / 11.0, 200)
+    out = {"workload": f"{nbox}_comoving_boxes_sizes32-192_{steps}steps", "boxes": nbox,
+           "cells": int(cells), "steps": steps, "size_histogram": {}}
+    for n, _, _, _ in spec:
+        out["size_histogram"][str(n)] = out["size_histogram"].get(str(n), 0) + 1
+    finals = {}
+    for prec in ("fp64", "mixed"):
+        tune_steps = 11
+        with batch.Batch(max_boxes=nbox, hbm_budget=200 << 30) as b:
+            ids = [b.add(st, nsteps=steps + tune_steps, cosmo=hp.cosmo_init(1.0 / 51.0, 0.8, ca, tstop),
+                         precision=prec) for st in states]
+            best, tune_ms, nrun = b.autotune(steps_per_trial=1)
+            b.use_graph(True)
+            b.step(1)                                   # graph capture, untimed
+            b.sync()
+            lat = []
+            t0 = time.perf_counter()
+            for _ in range(steps - 1):
+                t1 = time.perf_counter()
+                b.step(1)
+                b.sync()
+                lat.append(time.perf_counter() - t1)
+            wall = time.perf_counter() - t0
+            st = b.poll()
+            assert all(s.steps_done == nrun + steps for s in st.values())
+            finals[prec] = [b.download(i) for i in ids]
+        out[prec] = {"cell_updates_per_s": round(cells * (steps - 1) / wall, 1),
+                     "ms_per_batched_step": round(wall / (steps - 1) * 1e3, 3),
+                     "step_latency_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                     "step_latency_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                     "tuning": best, "autotune_steps": nrun}
+    worst = {}
+    for a, b_ in zip(finals["mixed"], finals["fp64"]):
+        e = errors(a, b_)
+        for k, v in e.items():
+            w = worst.setdefault(k, {"l1": 0.0, "linf": 0.0})
+            w["l1"] = max(w["l1"], v["l1"])
+            w["linf"] = max(w["linf"], v["linf"])
+    out["mixed_vs_fp64_worst_box_err"] = worst
+    return out
+
+
+def run_serving(nreq=48, rate=8.0, seed=5150):
+    """Poisson stream of box evolutions through the continuous-batching loop."""
+    from hydropad_amd import batch
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(nreq):
+        t += rng.exponential(1.0 / rate)
+        n = int(round(np.exp(rng.uniform(np.log(32), np.log(256)))))
+        reqs.append({"n": n, "nsteps": int(rng.integers(20, 201)), "precision": "mixed" if rng.random() < 0.5
+                     else "fp64", "seed": int(rng.integers(1 << 62)), "arrival_s": t})
+    res, stats, _ = batch.serve(reqs, hbm_budget=64 << 30, max_boxes=64, use_graph=True)
+    lat = np.array([r["done_s"] - r["arrival_s"] for r in res if r["rc"] == 0])
+    return {"workload": f"poisson_{rate}req/s_{nreq}req_N32-256_steps20-200_sod_seeded",
+            "requests": nreq, "rate_per_s": rate, "completed": int(stats["completed"]),
+            "cell_updates_per_s": round(stats["cell_updates"] / stats["wall_s"], 1),
+            "wall_s": round(stats["wall_s"], 3), "p50_latency_s": round(stats["p50_s"], 4),
+            "p99_latency_s": round(stats["p99_s"], 4), "mean_latency_s": round(float(lat.mean()), 4),
+            "batched_steps": int(stats["batched_steps"]), "max_concurrent": int(stats["max_concurrent"]),
+            "peak_hbm_gb": round(stats["peak_bytes"] / 1e9, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +287,7 @@ def main():
     ap.add_argument("--cour", type=float, default=0.8)
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -223,6 +338,13 @@ def main():
     }
     if "fp64" in results and head != "fp64":
         line["roofline_fp64"] = roofline_of(results["fp64"], N)
+    if not args.no_extras:
+        log("bench: cosmological 256^3")
+        line["cosmo_256"] = run_cosmo(hp, ic)
+        log("bench: ensemble")
+        line["ensemble"] = run_ensemble(hp, ic)
+        log("bench: serving")
+        line["serving"] = run_serving()
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline(N)
