@@ -68,7 +68,7 @@ EXPORTS = ("hp_params_init", "hp_create", "hp_destroy", "hp_set_precision", "hp_
            "hp_download", "hp_upload_device", "hp_download_device", "hp_set_params",
            "hp_ppmsolver", "hp_exchange_mesh", "hp_ppm_lxlylz", "hp_speed", "hp_evolve",
            "hp_status", "hp_reset_counters", "hp_sync", "hp_integrate_cell", "hp_dualenergy_cell",
-           "hp_sweep_timing_enable", "hp_sweep_timing", "hp_cosmo_init", "hp_timedet_cosmo",
+           "hp_sweep_timing_enable", "hp_sweep_timing", "hp_set_tiling", "hp_cosmo_init", "hp_timedet_cosmo",
            "hp_cosmo_params", "hp_savetn", "hp_expand", "hp_evolve_cosmo",
            "hp_batch_box_bytes", "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_add_sod",
            "hp_sod_of_seed", "hp_batch_remove", "hp_batch_download", "hp_batch_step", "hp_batch_sync",
@@ -109,6 +109,7 @@ def lib():
     L.hp_dualenergy_cell.argtypes = [PP] + [_D] * 8 + [_PD, _PD]
     L.hp_sweep_timing_enable.argtypes = [_P, _I]
     L.hp_sweep_timing.argtypes = [_P, _PD, ctypes.POINTER(ctypes.c_int64)]
+    L.hp_set_tiling.argtypes = [_P, _I, _I, _I]
     PC = ctypes.POINTER(HpCosmo)
     L.hp_cosmo_init.argtypes = [_D, _D, _D, _D, PC]
     L.hp_timedet_cosmo.argtypes = [PC, _D, _D]

@@ -11,11 +11,11 @@
 namespace hp {
 
 constexpr int BATCH_MAXB = 1024;          // boxes per batch
-constexpr int BATCH_NTABLE = 36;          // 3 slots x 3 directions x 2 precisions x 2 (expansion)
+constexpr int BATCH_NTABLE = 72;          // 3 slots x 3 directions x 2 precisions x expansion x gamma 5/3
 
-__host__ __device__ constexpr int table_key(int slot, int dir, int prec, int exp)
+__host__ __device__ constexpr int table_key(int slot, int dir, int prec, int exp, int g53)
 {
-    return ((slot * 3 + (dir - 1)) * 2 + prec) * 2 + exp;
+    return (((slot * 3 + (dir - 1)) * 2 + prec) * 2 + exp) * 2 + g53;
 }
 
 struct BatchBox {
@@ -25,6 +25,7 @@ struct BatchBox {
     int nx, ny, nz, nb;
     int nsets;            // 2, or 3 when the step keeps its start state for the expansion
     int prec;             // PREC_FP64 or PREC_MIXED
+    int g53;              // gamma = 5/3 scalars: the constant-folded kernels
     int active;           // slot in use
     int xcpu;             // x sweep: chunks of 58 cells per work unit
     int seglen[2];        // y, z sweeps: streaming segment length
@@ -72,6 +73,6 @@ __host__ __device__ inline int batch_current(int nsets, long long n)
 }
 
 hipError_t launch_batch_sweep(const BatchBox* boxes, const BatchPlan* plan, int slot, int dir, int prec, int exp,
-                              int grid, hipStream_t s);
+                              int g53, int grid, hipStream_t s);
 
 }  // namespace hp

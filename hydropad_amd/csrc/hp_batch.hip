@@ -39,7 +39,7 @@ __global__ void plan_kernel(const hp::BatchBox* __restrict__ boxes, hp::BatchPla
 {
     const int key = threadIdx.x;
     if (key >= hp::BATCH_NTABLE) return;
-    const int exp = key & 1, prec = (key >> 1) & 1, sd = key >> 2;
+    const int g53 = key & 1, exp = (key >> 1) & 1, prec = (key >> 2) & 1, sd = key >> 3;
     const int slot = sd / 3, dir = sd % 3 + 1;
     hp::BatchTable& T = plan->t[key];
     const int nbox = plan->nbox;
@@ -51,7 +51,7 @@ __global__ void plan_kernel(const hp::BatchBox* __restrict__ boxes, hp::BatchPla
         const int nrot = (step == 1) ? 1 : (int)(step % 6) + 1;
         const int d = ORDER[nrot - 1][slot];
         const int e = (slot == 2 && X.nsets == 3) ? 1 : 0;
-        if (d != dir || X.prec != prec || e != exp) continue;
+        if (d != dir || X.prec != prec || e != exp || X.g53 != g53) continue;
         T.box[n] = b;
         T.start[n] = total;
         total += X.units[dir - 1];
@@ -196,13 +196,13 @@ void set_units(hp::BatchBox& X, const hp_batch_tuning& t)
     }
 }
 
-// bit (prec*2 + comoving) for every active box
+// bit (prec*4 + comoving*2 + g53) for every active box
 unsigned combos_of(const hp_batch* b)
 {
     unsigned m = 0;
     for (int i = 0; i < b->nbox; ++i) {
         const hp::BatchBox& X = b->h_boxes[i];
-        if (X.active) m |= 1u << (X.prec * 2 + (X.nsets == 3 ? 1 : 0));
+        if (X.active) m |= 1u << (X.prec * 4 + (X.nsets == 3 ? 2 : 0) + X.g53);
     }
     return m;
 }
@@ -226,18 +226,22 @@ int push_nbox(hp_batch* b)
 
 int enqueue_step(hp_batch* b, unsigned combos, int grid)
 {
-    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(64), 0, b->stream, b->d_boxes, b->d_plan);
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(128), 0, b->stream, b->d_boxes, b->d_plan);
     for (int slot = 0; slot < 3; ++slot) {
         for (int prec = 0; prec < 2; ++prec) {
-            const bool stat = combos & (1u << (prec * 2)), como = combos & (1u << (prec * 2 + 1));
-            if (!stat && !como) continue;
-            for (int dir = 1; dir <= 3; ++dir) {
-                if (slot < 2 || stat)
-                    if (hp::launch_batch_sweep(b->d_boxes, b->d_plan, slot, dir, prec, 0, grid, b->stream) != hipSuccess)
-                        return HP_E_HIP;
-                if (slot == 2 && como)
-                    if (hp::launch_batch_sweep(b->d_boxes, b->d_plan, slot, dir, prec, 1, grid, b->stream) != hipSuccess)
-                        return HP_E_HIP;
+            for (int g = 0; g < 2; ++g) {
+                const bool stat = combos & (1u << (prec * 4 + g)), como = combos & (1u << (prec * 4 + 2 + g));
+                if (!stat && !como) continue;
+                for (int dir = 1; dir <= 3; ++dir) {
+                    if (slot < 2 || stat)
+                        if (hp::launch_batch_sweep(b->d_boxes, b->d_plan, slot, dir, prec, 0, g, grid, b->stream) !=
+                            hipSuccess)
+                            return HP_E_HIP;
+                    if (slot == 2 && como)
+                        if (hp::launch_batch_sweep(b->d_boxes, b->d_plan, slot, dir, prec, 1, g, grid, b->stream) !=
+                            hipSuccess)
+                            return HP_E_HIP;
+                }
             }
         }
     }
@@ -345,6 +349,7 @@ int start_box(hp_batch* b, int id, double gamma, double cour, double tstop, cons
         hipLaunchKernelGGL(first_dt_kernel, dim3(1), dim3(64), 0, b->stream, X.prm, X.st);
         HPCHK(hipGetLastError());
     }
+    X.g53 = hp::params_g53(*(const hp::Params*)&P) ? 1 : 0;
     X.active = 1;
     if (push_box(b, id)) return HP_E_HIP;
     HPCHK(hipStreamSynchronize(b->stream));       // host state above lives on this stack

@@ -46,7 +46,7 @@ __device__ __forceinline__ void batch_view(const BatchBox& X, int slot, BoxView&
     E.do_speed = slot == 2;
 }
 
-template <class R, bool EXP>
+template <class M, class R, bool EXP>
 __global__ void __launch_bounds__(256) batch_x(const BatchBox* __restrict__ boxes, const BatchPlan* __restrict__ plan,
                                                int key, int slot)
 {
@@ -67,14 +67,14 @@ __global__ void __launch_bounds__(256) batch_x(const BatchBox* __restrict__ boxe
         const unsigned nchunks = x_chunks(X.nx, X.ny, X.nz, X.nb);
         const unsigned c0 = (unsigned)local * (unsigned)X.xcpu;
         const unsigned c1 = min(c0 + (unsigned)X.xcpu, nchunks);
-        const AllScalars<R> A = load_scalars<R>(X.prm);
+        const AllScalars<R> A = load_scalars<M, R>(X.prm);
         Tally t = tally0();
-        x_run<FastMath, R, false, true, EXP>(V, A, E, c0, c1, t);
+        x_run<M, R, false, true, EXP>(V, A, E, c0, c1, t);
         flush_counters(X.st, t, slot == 2);
     }
 }
 
-template <class R, int DIR, bool EXP>
+template <class M, class R, int DIR, bool EXP>
 __global__ void __launch_bounds__(256) batch_stream(const BatchBox* __restrict__ boxes,
                                                     const BatchPlan* __restrict__ plan, int key, int slot)
 {
@@ -101,34 +101,43 @@ __global__ void __launch_bounds__(256) batch_stream(const BatchBox* __restrict__
         const long it = active ? item : 0;
         const int seg = (int)(it / npen);
         const long pen = it - (long)seg * npen;
-        const AllScalars<R> A = load_scalars<R>(X.prm);
+        const AllScalars<R> A = load_scalars<M, R>(X.prm);
         Tally t = tally0();
-        stream_run<FastMath, R, DIR, false, true, EXP>(V, A, E, pen, seg, X.seglen[DIR - 2], active, t);
+        stream_run<M, R, DIR, false, true, EXP>(V, A, E, pen, seg, X.seglen[DIR - 2], active, t);
         flush_counters(X.st, t, slot == 2);
     }
 }
 
-template <class R, bool EXP>
+template <class M, class R, bool EXP>
 static void launch_dir(const BatchBox* boxes, const BatchPlan* plan, int key, int slot, int dir, int grid,
                        hipStream_t s)
 {
-    if (dir == 1) hipLaunchKernelGGL((batch_x<R, EXP>), dim3(grid), dim3(256), 0, s, boxes, plan, key, slot);
-    else if (dir == 2) hipLaunchKernelGGL((batch_stream<R, 2, EXP>), dim3(grid), dim3(256), 0, s, boxes, plan, key, slot);
-    else hipLaunchKernelGGL((batch_stream<R, 3, EXP>), dim3(grid), dim3(256), 0, s, boxes, plan, key, slot);
+    if (dir == 1) hipLaunchKernelGGL((batch_x<M, R, EXP>), dim3(grid), dim3(256), 0, s, boxes, plan, key, slot);
+    else if (dir == 2)
+        hipLaunchKernelGGL((batch_stream<M, R, 2, EXP>), dim3(grid), dim3(256), 0, s, boxes, plan, key, slot);
+    else hipLaunchKernelGGL((batch_stream<M, R, 3, EXP>), dim3(grid), dim3(256), 0, s, boxes, plan, key, slot);
+}
+
+template <class M>
+static void launch_m(const BatchBox* boxes, const BatchPlan* plan, int key, int slot, int dir, int prec, int exp,
+                     int grid, hipStream_t s)
+{
+    if (prec == PREC_MIXED) {
+        if (exp) launch_dir<M, float, true>(boxes, plan, key, slot, dir, grid, s);
+        else launch_dir<M, float, false>(boxes, plan, key, slot, dir, grid, s);
+    } else {
+        if (exp) launch_dir<M, double, true>(boxes, plan, key, slot, dir, grid, s);
+        else launch_dir<M, double, false>(boxes, plan, key, slot, dir, grid, s);
+    }
 }
 
 hipError_t launch_batch_sweep(const BatchBox* boxes, const BatchPlan* plan, int slot, int dir, int prec, int exp,
-                              int grid, hipStream_t s)
+                              int g53, int grid, hipStream_t s)
 {
     if (slot < 0 || slot > 2 || dir < 1 || dir > 3 || grid < 1) return hipErrorInvalidValue;
-    const int key = table_key(slot, dir, prec, exp);
-    if (prec == PREC_MIXED) {
-        if (exp) launch_dir<float, true>(boxes, plan, key, slot, dir, grid, s);
-        else launch_dir<float, false>(boxes, plan, key, slot, dir, grid, s);
-    } else {
-        if (exp) launch_dir<double, true>(boxes, plan, key, slot, dir, grid, s);
-        else launch_dir<double, false>(boxes, plan, key, slot, dir, grid, s);
-    }
+    const int key = table_key(slot, dir, prec, exp, g53);
+    if (g53) launch_m<FastMath53>(boxes, plan, key, slot, dir, prec, exp, grid, s);
+    else launch_m<FastMath>(boxes, plan, key, slot, dir, prec, exp, grid, s);
     return hipGetLastError();
 }
 

@@ -83,6 +83,7 @@ struct hp_ctx {
     hipStream_t stream;
     int device;
     int seglen;
+    int xcpw;
     int block;
     bool timing;
     std::vector<hipEvent_t> ev_pool;     // recycled events
@@ -150,10 +151,12 @@ static int sweep(hp_ctx* c, int dir, int ghost_mask, int do_speed, bool expand =
     L.ghost_mask = ghost_mask;
     L.do_speed = do_speed;
     L.seglen = c->seglen;
+    L.xcpw = c->xcpw;
     L.block = c->block;
     L.precision = c->precision;
     L.grav = c->has_grav;
     L.cosmo = is_cosmo(c->h_prm);
+    L.g53 = hp::params_g53(*(const hp::Params*)&c->h_prm);
     hp::BoxView b = view_of(c, out);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
@@ -212,6 +215,7 @@ int hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out)
     c->precision = precision;
     c->ncell = (size_t)nx * ny * nz;
     c->seglen = 0;
+    c->xcpw = 0;
     c->block = 256;
     c->timing = false;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HP_E_HIP; }
@@ -466,6 +470,15 @@ int hp_evolve_cosmo(hp_ctx* c, int first_step, int nsteps, hp_cosmo* s)
     HPCHK(hipMemcpyAsync(&cs, &c->d_st->cosmo, sizeof(cs), hipMemcpyDeviceToHost, c->stream));
     HPCHK(hipStreamSynchronize(c->stream));
     memcpy(s, &cs, sizeof(cs));
+    return HP_OK;
+}
+
+int hp_set_tiling(hp_ctx* c, int x_chunks, int yz_seglen, int block)
+{
+    if (!c || x_chunks < 0 || yz_seglen < 0 || block < 0 || block > 256 || block % 64 != 0) return HP_E_ARG;
+    c->xcpw = x_chunks;
+    c->seglen = yz_seglen;
+    c->block = block > 0 ? block : 256;
     return HP_OK;
 }
 

@@ -50,14 +50,20 @@ struct BoxView {
 
 enum Precision { PREC_FP64 = 0, PREC_MIXED = 1, PREC_FP64_EXACT = 2 };
 
+// True when the scalars are gamma = 5/3 with exactly the derived constants
+// hp_params_init computes (then the constant-folded kernels give identical results).
+bool params_g53(const Params& P);
+
 struct SweepLaunch {
     int dir;          // 1,2,3
     int ghost_mask;   // bit d-1: also write the periodic face-ghost images along axis d
     int do_speed;     // fuse the CFL maximum into this sweep
     int seglen;       // streaming segment length for y/z sweeps (<= 0: whole pencil)
+    int xcpw;         // x sweep: chunks per wave (<= 0: about 16 resident rounds)
     int block;        // threads per block
     int precision;    // Precision
     bool grav, cosmo;
+    bool g53;         // gamma == 5/3 with the derived scalars of hp_params_init: constant-folded kernels
     // expansion step fused into this sweep (hydropad/expand.F90:18-41):
     // start-of-step p, vx, vy, vz, or old[0] == nullptr; fac -> device
     // (afact, 1/(1+afact), 1/(2+afact))

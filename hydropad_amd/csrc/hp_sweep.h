@@ -15,6 +15,7 @@
 // periodic face-ghost images of what they store (the one-rank exchange_mesh,
 // hydropad/exchange_mesh.F90:47-123) in the same pass.
 #pragma once
+#include <algorithm>
 #include "hp_core.h"
 #include "hp_fast.h"
 #include "hp_kernels.h"
@@ -28,6 +29,7 @@ template <class R> struct AllScalars;
 
 // Reference-order arithmetic (hp_core.h); R must be double.
 struct ExactMath {
+    static constexpr bool G53 = false;
     template <class R> static __device__ __forceinline__ R slope(double am1, double a, double ap1)
     {
         return ajside(am1, a, ap1);
@@ -58,6 +60,7 @@ struct ExactMath {
 
 // Throughput forms (hp_fast.h), delta-form reconstruction.
 struct FastMath {
+    static constexpr bool G53 = false;
     template <class R> static __device__ __forceinline__ R slope(double am1, double a, double ap1)
     {
         return fm::slope_d<R>(am1, a, ap1);
@@ -87,6 +90,14 @@ struct FastMath {
     {
         return fm::fsqrt(gamma * p * fm::rcp(rho));
     }
+};
+
+// The throughput forms with gamma = 5/3 fixed at compile time: the
+// gamma-derived scalars become constants (same values, bit for bit, as
+// hp_params_init computes them from 5.0/3.0), the Riemann solver's powers are
+// the root/product forms only, and the freed registers raise occupancy.
+struct FastMath53 : FastMath {
+    static constexpr bool G53 = true;
 };
 
 // field index (0 p, 1 rho, 2 vx, 3 vy, 4 vz) of the normal velocity and the
@@ -152,22 +163,42 @@ template <class R> struct AllScalars {
     double rgamma1, gamma, mingradflat;
 };
 
-template <class R> __device__ __forceinline__ AllScalars<R> load_scalars(const Params* __restrict__ prm)
+// gamma-derived constants for gamma = 5/3, by the expressions of
+// hydropad/initialize_parameters.F90:64-69 (and hp_params_init)
+struct Gamma53 {
+    static constexpr double gamma = 5.0 / 3.0;
+    static constexpr double gamma1 = (gamma + 1.0) / (gamma - 1.0);
+    static constexpr double rgamma = 1.0 / gamma;
+    static constexpr double rgamma1 = 1.0 / (gamma - 1);
+    static constexpr double m = (gamma - 1.0) / (2.0 * gamma);
+    static constexpr double rm = 1.0 / m;
+    static constexpr double gf = (gamma + 1.0) / (2.0 * gamma);
+};
+
+template <class M, class R> __device__ __forceinline__ AllScalars<R> load_scalars(const Params* __restrict__ prm)
 {
     const Params P = *prm;
+    double gamma = P.gamma, gamma1 = P.gamma1, rgamma = P.rgamma, rgamma1 = P.rgamma1;
+    double m = P.m, rm = P.rm, gf = P.gf;
+    int m5 = fabs(P.m - 0.2) < 1e-15;
+    if constexpr (M::G53) {
+        gamma = Gamma53::gamma; gamma1 = Gamma53::gamma1; rgamma = Gamma53::rgamma; rgamma1 = Gamma53::rgamma1;
+        m = Gamma53::m; rm = Gamma53::rm; gf = Gamma53::gf;
+        m5 = 1;
+    }
     AllScalars<R> A;
-    A.S.gamma = (R)P.gamma; A.S.dt = (R)P.dt; A.S.rat = (R)P.rat; A.S.dx = (R)P.dx;
+    A.S.gamma = (R)gamma; A.S.dt = (R)P.dt; A.S.rat = (R)P.rat; A.S.dx = (R)P.dx;
     A.S.rdx = (R)P.rdx; A.S.dat = (R)P.dat; A.S.at = (R)P.at;
     A.S.flatvalue = (R)P.flatvalue; A.S.mingradflat = (R)P.mingradflat;
     A.S.dtrat = (R)(P.dt * P.rat);
-    A.Rs.gamma = P.gamma; A.Rs.gamma1 = P.gamma1; A.Rs.rgamma = P.rgamma; A.Rs.rgamma1 = P.rgamma1;
-    A.Rs.m = P.m; A.Rs.rm = P.rm; A.Rs.gf = P.gf;
+    A.Rs.gamma = gamma; A.Rs.gamma1 = gamma1; A.Rs.rgamma = rgamma; A.Rs.rgamma1 = rgamma1;
+    A.Rs.m = m; A.Rs.rm = rm; A.Rs.gf = gf;
     A.Rs.dadt = 1.0 - 0.5 * P.dat * P.dt * P.rat;     // riemannsolver.F90:36
     A.Rs.dmax = P.dmax;
-    A.Rs.m5 = fabs(P.m - 0.2) < 1e-15;
-    A.Is.rdtath = P.rdtath; A.Is.rdx = P.rdx; A.Is.gamma = P.gamma; A.Is.rgamma1 = P.rgamma1;
+    A.Rs.m5 = m5;
+    A.Is.rdtath = P.rdtath; A.Is.rdx = P.rdx; A.Is.gamma = gamma; A.Is.rgamma1 = rgamma1;
     A.Is.eta1 = P.eta1; A.Is.eta2 = P.eta2;
-    A.rgamma1 = P.rgamma1; A.gamma = P.gamma; A.mingradflat = P.mingradflat;
+    A.rgamma1 = rgamma1; A.gamma = gamma; A.mingradflat = P.mingradflat;
     return A;
 }
 
@@ -451,7 +482,7 @@ __global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, Epilogue E, int c
     const unsigned c0 = wave * (unsigned)chunks_per_wave;
     if (c0 >= nchunks) return;                         // whole wave past the end (uniform)
     const unsigned c1 = min(c0 + (unsigned)chunks_per_wave, nchunks);
-    const AllScalars<R> A = load_scalars<R>(B.prm);
+    const AllScalars<R> A = load_scalars<M, R>(B.prm);
     Tally T = tally0();
     x_run<M, R, GRAV, COSMO, EXP>(B, A, E, c0, c1, T);
     flush_counters(B.st, T, E.do_speed != 0);
@@ -586,7 +617,7 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, Epilogue E, int s
     const long tt = active ? tid : 0;
     const int seg = (int)(tt / npen);
     const long pen = tt - (long)seg * npen;
-    const AllScalars<R> A = load_scalars<R>(B.prm);
+    const AllScalars<R> A = load_scalars<M, R>(B.prm);
     Tally T = tally0();
     stream_run<M, R, DIR, GRAV, COSMO, EXP>(B, A, E, pen, seg, seglen, active, T);
     flush_counters(B.st, T, E.do_speed != 0);
@@ -611,7 +642,7 @@ static hipError_t launch_sweep_t(const BoxView& b, const SweepLaunch& L, hipStre
         // about 16 resident rounds of waves over 256 CUs x 4 SIMDs; each wave
         // walks its run of chunks with a one-chunk register prefetch
         long cpw = nchunks / 16384;
-        if (L.seglen > 0) cpw = L.seglen;
+        if (L.xcpw > 0) cpw = L.xcpw;
         if (cpw < 1) cpw = 1;
         const long waves = (nchunks + cpw - 1) / cpw;
         const long threads = waves * 64;
@@ -620,7 +651,13 @@ static hipError_t launch_sweep_t(const BoxView& b, const SweepLaunch& L, hipStre
     } else {
         const int Nax = (L.dir == 2) ? Ny : Nz;
         const int Nbb = (L.dir == 2) ? Nz : Ny;
-        int seglen = L.seglen > 0 ? L.seglen : Nax;
+        // default: split pencils into segments of at least 64 cells until there
+        // are ~16K waves (each segment recomputes a 6-cell halo)
+        int seglen = L.seglen;
+        if (seglen <= 0) {
+            const long npen = (long)Nx * ((L.dir == 2) ? Nz : Ny);
+            seglen = (int)std::min<long>(Nax, std::max<long>(64, (long)Nax * npen / (16384L * 64)));
+        }
         if (seglen > Nax) seglen = Nax;
         const int nseg = (Nax + seglen - 1) / seglen;
         const long threads = (long)Nx * Nbb * nseg;

@@ -169,6 +169,10 @@ class Box:
         check("hp_cosmo_params", self.L.hp_cosmo_params(ctypes.byref(sched), ctypes.byref(self.params)))
         return sched
 
+    def set_tiling(self, x_chunks=0, yz_seglen=0, block=0):
+        """Launch shape of the sweeps (never changes results)."""
+        check("hp_set_tiling", self.L.hp_set_tiling(self._h, int(x_chunks), int(yz_seglen), int(block)))
+
     def sweep_timing_enable(self, on=True):
         check("hp_sweep_timing_enable", self.L.hp_sweep_timing_enable(self._h, 1 if on else 0))
 

@@ -166,6 +166,15 @@ int hp_expand(hp_ctx* ctx, double afact);
  * return. */
 int hp_evolve_cosmo(hp_ctx* ctx, int first_step, int nsteps, hp_cosmo* s);
 
+/* Launch shape of this context's sweeps: x_chunks = 58-cell chunks each
+ * x-sweep wave walks (0: about 16 resident rounds of waves), yz_seglen =
+ * y/z pencil segment length (0: at least 64 cells, short enough for ~16K
+ * waves), block = threads per block
+ * (0: 256; a multiple of 64, at most 256).  The shape never changes a result.
+ * No reference counterpart (the reference's OpenACC gang/vector shapes are
+ * compiler-chosen). */
+int hp_set_tiling(hp_ctx* ctx, int x_chunks, int yz_seglen, int block);
+
 /* Per-sweep kernel timing with HIP events on the context's stream; replaces
  * the MPI_WTIME "Elapsed time" print around every ppmsolver call
  * (hydro/ppmsolver.F90:72,464-465).  While enabled, every sweep launch is
