@@ -134,11 +134,12 @@ __device__ __forceinline__ float pow_pos(float x, float y)
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
 }
 
-// x^(1/5) for x >= 0: FP32 seed, then two Newton steps on y^5 = x,
-// y <- (4y + x/y^4)/5 (each step squares the relative error).  The seed needs
-// x inside FP32's normal range; near-vacuum rarefactions produce pressure
-// ratios far below it (p*/p ~ 1e-45 and less), so outside [1e-30, 1e30] the
-// argument is first scaled by 2^(-5q) and the root by 2^q.  x = 0 gives 0.
+// x^(1/5) for x >= 0: FP32 seed of y = x^(-1/5), two Newton steps on
+// y^-5 = x, y <- y (6 - x y^5)/5 (no division; each step takes the relative
+// error e to about 3e^2), then x^(1/5) = x y^4.  The seed needs x inside
+// FP32's normal range; near-vacuum rarefactions produce pressure ratios far
+// below it (p*/p ~ 1e-45 and less), so outside [1e-30, 1e30] the argument is
+// first scaled by 2^(-5q) and the root by 2^q.  x = 0 gives 0.
 __device__ __forceinline__ double root5(double x)
 {
     double xs = x, sc = 1.0;
@@ -149,12 +150,13 @@ __device__ __forceinline__ double root5(double x)
         xs = __builtin_amdgcn_ldexp(x, -5 * q);
         sc = __builtin_amdgcn_ldexp(1.0, q);
     }
-    double y = (double)__builtin_amdgcn_exp2f(0.2f * __builtin_amdgcn_logf((float)xs));
+    double y = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)xs));
     double y2 = y * y;
-    y = fma(0.8, y, 0.2 * xs * rcp(y2 * y2));
+    y = y * fma(-0.2 * xs, y2 * y2 * y, 1.2);
     y2 = y * y;
-    y = fma(0.8, y, 0.2 * xs * rcp(y2 * y2));
-    return y * sc;
+    y = y * fma(-0.2 * xs, y2 * y2 * y, 1.2);
+    y2 = y * y;
+    return xs * (y2 * y2) * sc;
 }
 
 // The three powers the Riemann solver takes: x^m, x^(1/gamma), x^(1/m).  For
@@ -385,47 +387,74 @@ __device__ __forceinline__ bool pcheck_guess(const RScal& Rs, double pleft, doub
     return false;
 }
 
+// 1/sqrt(x) and sqrt(x) for x > 0 together: v_rsq_f64 seed and two Newton
+// steps on 1/sqrt, y <- y (1.5 - 0.5 x y^2).
+__device__ __forceinline__ void rsq_sqrt(double x, double& rs, double& sq)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * (x * y), y, 1.5);
+    y = y * fma(-0.5 * (x * y), y, 1.5);
+    rs = y;
+    sq = x * y;
+}
+
+// One side of the van Leer iteration at pstar (riemann.F90:70-97): the inverse
+// wave impedance 1/a and the left side's form of dv*/dp* (the right side's is
+// its negative).  shock = pstar >= p_side; cl2 = c^2, rc = 1/c for the side's
+// Lagrangian sound speed c; x = pstar/p_side, xinv = p_side/pstar.
+__device__ __forceinline__ void newton_side(const RScal& Rs, bool shock, double x, double xinv, double cl2,
+                                            double rc, double& ra, double& vp)
+{
+    const double m = Rs.m, gf = Rs.gf;
+    double w = 1.0;
+    if (!shock) {
+        w = pow_m(Rs, x);
+        shock = (w == 1.0);                          // pstup == 0: the shock impedance (:74-75)
+    }
+    if (shock) {                                     // a = c sqrt(gf x + m), v' = -(a^2 + c^2)/(2 a^3)
+        const double q = fma(gf, x, m);
+        double rs, sq;
+        rsq_sqrt(q, rs, sq);
+        ra = rc * rs;
+        vp = -(cl2 * q + cl2) * 0.5 * (ra * ra * ra);
+    } else {                                         // a = m c (1 - x)/(1 - x^m), v' = -(1/c) x^m / x
+        ra = (1.0 - w) * rcp(m * (1.0 - x)) * rc;
+        vp = -rc * w * xinv;
+    }
+}
+
 // van Leer Newton iteration in FP64 (riemann.F90:58-109) from the initial
-// guess in pstar; the same iterates as the reference up to rounding.
+// guess in pstar; the same iterates as the reference up to rounding.  vstar is
+// the reference's (pl - pr + a vl + b vr)/(a + b) with the last iteration's
+// impedances, written with 1/a and 1/b.
 __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vstar, double pleft, double pright,
                                       double rholeft, double rhoright, double vxleft, double vxright)
 {
-    const double gamma = Rs.gamma, m = Rs.m, gf = Rs.gf;
-    const double cleft = fsqrt(gamma * pleft * rholeft);
-    const double cright = fsqrt(gamma * pright * rhoright);
-    const double rpl = rcp(pleft), rpr = rcp(pright);
-    const double rcl = rcp(cleft), rcr = rcp(cright);
+    const double gamma = Rs.gamma;
+    double rcl, cleft, rcr, cright;
+    rsq_sqrt(gamma * pleft * rholeft, rcl, cleft);
+    rsq_sqrt(gamma * pright * rhoright, rcr, cright);
     const double cl2 = cleft * cleft, cr2 = cright * cright;
+    const double rpl = rcp(pleft), rpr = rcp(pright);
+    double ra = 0.0, rb = 0.0;
     int nnewt = 0;
     for (;;) {
-        if (++nnewt > 100) return -1;
+        if (++nnewt > 100) {                         // where the reference stops (riemann.F90:60-65)
+            nnewt = -1;
+            break;
+        }
         const double pstarold = pstar;
-        double a, b, vpl, vpr;
-        const double xl = pstar * rpl, xr = pstar * rpr;
-        if (pstar >= pleft) {
-            a = cleft * fsqrt(gf * xl + m);
-            vpl = -(a * a + cl2) * 0.5 * rcp(a * a * a);
-        } else {
-            const double w = pow_m(Rs, xl);
-            a = (w == 1.0) ? cleft * fsqrt(gf * xl + m) : m * cleft * div(1.0 - xl, 1.0 - w);
-            vpl = -rcl * div(w, xl);
-        }
-        if (pstar >= pright) {
-            b = cright * fsqrt(gf * xr + m);
-            vpr = (b * b + cr2) * 0.5 * rcp(b * b * b);
-        } else {
-            const double w = pow_m(Rs, xr);
-            b = (w == 1.0) ? cright * fsqrt(gf * xr + m) : m * cright * div(1.0 - xr, 1.0 - w);
-            vpr = rcr * div(w, xr);
-        }
-        const double ra = rcp(a), rb = rcp(b);
+        const double rps = rcp(pstar);
+        double vpl, vpr;
+        newton_side(Rs, pstar >= pleft, pstar * rpl, pleft * rps, cl2, rcl, ra, vpl);
+        newton_side(Rs, pstar >= pright, pstar * rpr, pright * rps, cr2, rcr, rb, vpr);
         const double vstarl = vxleft - (pstar - pleft) * ra;
         const double vstarr = vxright + (pstar - pright) * rb;
-        pstar = pstar - div(vstarl - vstarr, vpl - vpr);
+        pstar = pstar - div(vstarl - vstarr, vpl + vpr);     // vprimel - vprimer = vpl + vpr
         if (pstar <= 0.0) pstar = pstarold * 1e-3;
-        vstar = div(pleft - pright + a * vxleft + b * vxright, a + b);
         if (!(fabs(pstarold - pstar) > K_TERR * pstarold)) break;
     }
+    vstar = div(fma((pleft - pright) * ra, rb, fma(vxleft, rb, vxright * ra)), ra + rb);
     return nnewt;
 }
 
@@ -462,7 +491,9 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
     if (pleft <= pright) { p1 = pleft; p2 = pright; rho1 = rholeft; c1 = cleft; c2 = cright; }
     else { p1 = pright; p2 = pleft; rho1 = rhoright; c1 = cright; c2 = cleft; }
     const R dvv = vxleft - vxright;
-    const R dv2s = div(p2 - p1, fsqrt(R(0.5) * rho1 * ((gamma - R(1)) * p1 + (gamma + R(1)) * p2)));
+    double rs2, sq2;
+    rsq_sqrt(R(0.5) * rho1 * ((gamma - R(1)) * p1 + (gamma + R(1)) * p2), rs2, sq2);
+    const R dv2s = (p2 - p1) * rs2;
     const R dv2r = -R(2) * (c1 + c2) * rgamma1;
     int kase;
     if (dvv >= dv2s) kase = 1;

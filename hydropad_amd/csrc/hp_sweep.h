@@ -86,10 +86,9 @@ struct FastMath {
     {
         fm::integrate_cell(S, F1, F2, p, rho, u, v, w, etot, g, grav);
     }
-    static __device__ __forceinline__ double vsound(double gamma, double p, double rho)
-    {
-        return fm::fsqrt(gamma * p * fm::rcp(rho));
-    }
+    // the CFL sound speed keeps speed.F90's IEEE sqrt(gamma*p/rho), so the
+    // fused maximum equals hp_speed's bit for bit and every path gets the same dt
+    static __device__ __forceinline__ double vsound(double gamma, double p, double rho) { return sqrt(gamma * p / rho); }
 };
 
 // The throughput forms with gamma = 5/3 fixed at compile time: the
