@@ -96,11 +96,11 @@ __global__ void __launch_bounds__(256) batch_stream(const BatchBox* __restrict__
         const int Nbb = (DIR == 2) ? X.nz - 2 * X.nb : X.ny - 2 * X.nb;
         const long npen = (long)(X.nx - 2 * X.nb) * Nbb;
         const int nseg = X.nseg[DIR - 2];
-        const long item = (long)local * 64 + lane;
-        const bool active = item < npen * nseg;
-        const long it = active ? item : 0;
-        const int seg = (int)(it / npen);
-        const long pen = it - (long)seg * npen;
+        const unsigned item = (unsigned)local * 64u + (unsigned)lane;   // items per box < 2^31
+        const bool active = (long)item < npen * nseg;
+        const unsigned it = active ? item : 0u;
+        const int seg = (int)(it / (unsigned)npen);
+        const long pen = (long)(it - (unsigned)seg * (unsigned)npen);
         const AllScalars<R> A = load_scalars<M, R>(X.prm);
         Tally t = tally0();
         stream_run<M, R, DIR, false, true, EXP>(V, A, E, pen, seg, X.seglen[DIR - 2], active, t);

@@ -50,6 +50,7 @@ enum { VP = 0, VR = 1, VU = 2, VV = 3, VW = 4, VG = 5 };
 template <class R> struct SweepScalars {
     R gamma, dt, rat, dx, rdx, dat, at, flatvalue, mingradflat;
     R dtrat;   // dt*rat, used only by the fast forms
+    R cdat2;   // 2*dat*rat*dt (expansion term of beta/beta0), used only by the fast forms
 };
 
 // ---------------------------------------------------------------------------

@@ -262,7 +262,7 @@ __device__ __forceinline__ R beta_d(const SweepScalars<R>& S, R rct, R ut, R uc,
                                     R gc_abs, R sign)
 {
     R sum = (ut - uc) + sign * (pt - pc) * rct;
-    if (COSMO) sum = sum + sign * S.dt * (R(2) * S.dat * pc_abs * rct / S.at);
+    if (COSMO) sum = sum + sign * (S.cdat2 * pc_abs * rct);            // dt 2 adot pc/(C a)
     if (GRAV || COSMO) {
         R gvc = ((GRAV ? -gc_abs : R(0)) + (COSMO ? S.dat * uc_abs : R(0))) * S.rat;
         sum = sum + S.dt * gvc;
@@ -331,7 +331,7 @@ __device__ __forceinline__ Face<double> trace_face_d(const SweepScalars<R>& S, c
         const R rct2 = rct * rct;
         const R rtr = (R)rtl, rcr = (R)a[VR] + rc2;
         b0 = (pt - pc2) * rct2 + div(rc2 - rt, rtr * rcr);
-        if (COSMO) b0 = b0 + (R(2) * S.dat * (ap + pc2)) * rct2 / S.at * S.dt;
+        if (COSMO) b0 = b0 + S.cdat2 * (ap + pc2) * rct2;                 // 2 adot pc dt/(a C^2)
     }
     Face<double> f;
     f.p = ptl + (double)(ct * ct * (bp + bm));
@@ -607,6 +607,29 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
 }
 
 // ---- conservative update + dual energy (FP64) ------------------------------------------
+// Dual-energy selection of integrate.F90:196-216 with the ratio tests
+// e_aux/E >= eta cross-multiplied when E > 0 (the reference's division
+// otherwise).
+__device__ __forceinline__ void dual_energy_d(const IScal& S, double etotnew, double rhonew, double vxnew,
+                                              double vynew, double vznew, double eint, double pres,
+                                              double& eintnew, double& pnew)
+{
+    const double vvv = vxnew * vxnew + vynew * vynew + vznew * vznew;
+    const double eeaux = etotnew - 0.50 * rhonew * vvv;
+    bool ge2, ge1;
+    if (etotnew > 0.0) {
+        ge2 = eeaux >= S.eta2 * etotnew;
+        ge1 = eeaux >= S.eta1 * etotnew;
+    } else {
+        const double ee1 = eeaux / etotnew;
+        ge2 = ee1 >= S.eta2;
+        ge1 = ee1 >= S.eta1;
+    }
+    if (ge2) eintnew = eeaux;
+    pnew = (S.gamma - 1) * (ge1 ? eeaux : eintnew);
+    if (eintnew <= 0.0) { eintnew = eint; pnew = pres; }
+}
+
 __device__ __forceinline__ void integrate_cell(const IScal& S, const Flux& F1, const Flux& F2, double& pres,
                                                double& rho, double& vx, double& vy, double& vz, double etot,
                                                double ghalf, bool grav)
@@ -629,7 +652,7 @@ __device__ __forceinline__ void integrate_cell(const IScal& S, const Flux& F1, c
     double eintnew = eint + lam * (pmean * (F1.u - F2.u) + (F1.p * F1.u - F2.p * F2.u) * rgamma1);
     if (eintnew < 0.0) eintnew = eint;
     double pnew;
-    dual_energy(S, etotnew, rhonew, vxnew, vynew, vznew, eint, pres, eintnew, pnew);
+    dual_energy_d(S, etotnew, rhonew, vxnew, vynew, vznew, eint, pres, eintnew, pnew);
     if (rhonew <= 0.0) rhonew = rho;
     pres = pnew;
     rho = rhonew;

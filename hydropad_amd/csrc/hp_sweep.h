@@ -190,6 +190,7 @@ template <class M, class R> __device__ __forceinline__ AllScalars<R> load_scalar
     A.S.rdx = (R)P.rdx; A.S.dat = (R)P.dat; A.S.at = (R)P.at;
     A.S.flatvalue = (R)P.flatvalue; A.S.mingradflat = (R)P.mingradflat;
     A.S.dtrat = (R)(P.dt * P.rat);
+    A.S.cdat2 = (R)(2.0 * P.dat * P.rat * P.dt);
     A.Rs.gamma = gamma; A.Rs.gamma1 = gamma1; A.Rs.rgamma = rgamma; A.Rs.rgamma1 = rgamma1;
     A.Rs.m = m; A.Rs.rm = rm; A.Rs.gf = gf;
     A.Rs.dadt = 1.0 - 0.5 * P.dat * P.dt * P.rat;     // riemannsolver.F90:36
@@ -509,7 +510,8 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
     int Nax, Nbb;
     long s_ax, s_b;
     stream_geometry<DIR>(B, Nax, Nbb, s_ax, s_b);
-    const int ia = (int)(pen % Nx), ib = (int)(pen / Nx);
+    const unsigned upen = (unsigned)pen;                 // pencils per box < 2^31
+    const int ib = (int)(upen / (unsigned)Nx), ia = (int)(upen - (unsigned)ib * (unsigned)Nx);
     const int s0 = seg * seglen;
     const int s1 = min(s0 + seglen, Nax);
     const long base = (nb + ia) + (long)(nb + ib) * s_b + (long)nb * s_ax;   // interior cell 0 of the pencil
@@ -611,11 +613,11 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, Epilogue E, int s
     long s_ax, s_b;
     stream_geometry<DIR>(B, Nax, Nbb, s_ax, s_b);
     const long npen = (long)(B.nx - 2 * B.nb) * Nbb;
-    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = tid < npen * nseg;
-    const long tt = active ? tid : 0;
-    const int seg = (int)(tt / npen);
-    const long pen = tt - (long)seg * npen;
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;   // items per sweep < 2^31
+    const bool active = (long)tid < npen * nseg;
+    const unsigned tt = active ? tid : 0u;
+    const int seg = (int)(tt / (unsigned)npen);
+    const long pen = (long)(tt - (unsigned)seg * (unsigned)npen);
     const AllScalars<R> A = load_scalars<M, R>(B.prm);
     Tally T = tally0();
     stream_run<M, R, DIR, GRAV, COSMO, EXP>(B, A, E, pen, seg, seglen, active, T);
