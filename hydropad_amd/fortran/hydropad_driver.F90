@@ -6,18 +6,27 @@
 ! (hydropad/main.F90:125-136 + io/write_var_mpi.F90: rho.bin, pressure.bin,
 ! vx.bin, vy.bin, vz.bin; REAL(4) interior, Fortran order, no header).
 !
-!   hydropad_driver N NSTEPS [fp64|mixed|exact] [OUTDIR]
+!   hydropad_driver N NSTEPS [fp64|mixed|exact] [OUTDIR] [resident|sweeps]
+!
+! "resident" keeps the state on the GPU for the whole run (attach /
+! ppm_lxlylz_gpu / speed_gpu / detach); "sweeps" calls the per-sweep drop-in
+! ppmsolver_gpu with host arrays exactly where hydro/ppm_lxlylz.F90 calls
+! ppmsolver, and computes speed on the host (hydropad/speed.F90:22-31).
 program hydropad_driver
   use iso_c_binding
   use hydropad_gpu
   implicit none
-  integer :: n, nsteps, nb, nx, ny, nz, nstep, nrot, i
+  integer :: n, nsteps, nb, nx, ny, nz, nstep, nrot, i, isweep
+  integer, parameter :: order(3, 6) = reshape([1,2,3, 3,2,1, 2,3,1, 1,3,2, 3,1,2, 2,1,3], [3, 6])  ! ppm_lxlylz.F90
   real(c_double), allocatable :: p3d(:,:,:), rho3d(:,:,:), vx3d(:,:,:), vy3d(:,:,:), vz3d(:,:,:)
+  real(c_double), allocatable :: gforce(:,:,:,:)
+  real(c_double) :: cho3d(1), nes3d(1)
+  character(len=16) :: mode
   real(c_double) :: gamma, gamma1, rgamma, rgamma1, m, rm, gf, dx, rdx, dt, t, cour, velmax, x, tstop
   real(c_double) :: flatvalue, mingradflat, eta1, eta2, dmax
   character(len=256) :: arg, outdir
 
-  n = 64; nsteps = 10; outdir = '.'
+  n = 64; nsteps = 10; outdir = '.'; mode = 'resident'
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg); read(arg, *) n
   end if
@@ -30,6 +39,7 @@ program hydropad_driver
     if (trim(arg) == 'exact') call hydropad_gpu_set_precision(HP_FP64_EXACT)
   end if
   if (command_argument_count() >= 4) call get_command_argument(4, outdir)
+  if (command_argument_count() >= 5) call get_command_argument(5, mode)
 
   nb = 3
   nx = n + 2*nb; ny = nx; nz = nx
@@ -60,21 +70,36 @@ program hydropad_driver
 
   call hydropad_gpu_set_scalars(mingradflat, flatvalue, gamma, 1.0d0, m, rgamma1, rm, rgamma, dmax, &
                                 rdx, dt, dt, dx, 0.0d0, 1.0d0, gamma1, gf, eta1, eta2)
-  call hydropad_gpu_attach(p3d, rho3d, vx3d, vy3d, vz3d, nx, ny, nz, nb)
-  call speed_gpu(velmax)
-  call timedet(velmax)                  ! the first dt from the initial state's CFL condition
-
-  do nstep = 1, nsteps                  ! hydropad/main.F90:100-121
-    nrot = mod(nstep, 6) + 1            ! hydropad/evolve_sys.F90:30-31
-    if (nstep == 1) nrot = 1
-    call hydropad_gpu_set_scalars(mingradflat, flatvalue, gamma, 1.0d0, m, rgamma1, rm, rgamma, dmax, &
-                                  rdx, dt / 1.0d0, dt, dx, 0.0d0, 1.0d0, gamma1, gf, eta1, eta2)
-    call ppm_lxlylz_gpu(nrot)
+  if (trim(mode) == 'sweeps') then
+    allocate(gforce(3, nx, ny, nz))
+    gforce = 0.0d0
+    call timedet(host_speed())          ! the first dt from the initial state's CFL condition
+    do nstep = 1, nsteps                ! hydropad/main.F90:100-121
+      nrot = mod(nstep, 6) + 1          ! hydropad/evolve_sys.F90:30-31
+      if (nstep == 1) nrot = 1
+      call hydropad_gpu_set_scalars(mingradflat, flatvalue, gamma, 1.0d0, m, rgamma1, rm, rgamma, dmax, &
+                                    rdx, dt / 1.0d0, dt, dx, 0.0d0, 1.0d0, gamma1, gf, eta1, eta2)
+      do isweep = 1, 3                  ! hydro/ppm_lxlylz.F90: ppmsolver + exchange_mesh per sweep
+        call ppmsolver_gpu(p3d, rho3d, vx3d, vy3d, vz3d, cho3d, nes3d, gforce, nx, ny, nz, nb, &
+                           order(isweep, nrot))
+      end do
+      call timedet(host_speed())
+    end do
+  else
+    call hydropad_gpu_attach(p3d, rho3d, vx3d, vy3d, vz3d, nx, ny, nz, nb)
     call speed_gpu(velmax)
-    call timedet(velmax)
-  end do
-
-  call hydropad_gpu_detach(p3d, rho3d, vx3d, vy3d, vz3d)
+    call timedet(velmax)                ! the first dt from the initial state's CFL condition
+    do nstep = 1, nsteps                ! hydropad/main.F90:100-121
+      nrot = mod(nstep, 6) + 1          ! hydropad/evolve_sys.F90:30-31
+      if (nstep == 1) nrot = 1
+      call hydropad_gpu_set_scalars(mingradflat, flatvalue, gamma, 1.0d0, m, rgamma1, rm, rgamma, dmax, &
+                                    rdx, dt / 1.0d0, dt, dx, 0.0d0, 1.0d0, gamma1, gf, eta1, eta2)
+      call ppm_lxlylz_gpu(nrot)
+      call speed_gpu(velmax)
+      call timedet(velmax)
+    end do
+    call hydropad_gpu_detach(p3d, rho3d, vx3d, vy3d, vz3d)
+  end if
   call write_var(rho3d, 'rho.bin'); call write_var(p3d, 'pressure.bin')
   call write_var(vx3d, 'vx.bin'); call write_var(vy3d, 'vy.bin'); call write_var(vz3d, 'vz.bin')
   write(*, '(a,i0,a,i0,a,es24.16,a,es24.16,a,es24.16)') 'hydropad_driver: N=', n, ' steps=', nsteps, &
@@ -82,6 +107,21 @@ program hydropad_driver
   call hydropad_gpu_finalize()
 
 contains
+
+  ! CFL maximum on the host, hydropad/speed.F90:22-31
+  real(c_double) function host_speed()
+    integer :: ii, jj, kk
+    real(c_double) :: cs
+    host_speed = 0.0d0
+    do kk = nb + 1, nz - nb
+      do jj = nb + 1, ny - nb
+        do ii = nb + 1, nx - nb
+          cs = sqrt(gamma * p3d(ii, jj, kk) / rho3d(ii, jj, kk))
+          host_speed = max(host_speed, max(abs(vx3d(ii, jj, kk)), abs(vy3d(ii, jj, kk)), abs(vz3d(ii, jj, kk))) + cs)
+        end do
+      end do
+    end do
+  end function host_speed
 
   ! hydropad/functions.F90:2-35 (non-cosmological)
   subroutine timedet(vmax)
