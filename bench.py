@@ -85,7 +85,7 @@ def errors(a, b):
     return res
 
 
-def roofline_of(r, N):
+def roofline_of(r, N, prec):
     names = ("sweep_lane_x", "sweep_stream<y>", "sweep_stream<z>")
     tot = [r["ms"][d] for d in range(3)]
     dom = int(np.argmax(tot))
@@ -96,7 +96,7 @@ def roofline_of(r, N):
     if os.path.exists(TRAFFIC_FILE):
         try:
             tr = json.load(open(TRAFFIC_FILE))
-            key = f"{names[dom]}@{N}"
+            key = f"{names[dom]}@{N}@{prec}"
             if key in tr:
                 traffic = tr[key]["bytes_per_launch"]
         except (ValueError, KeyError):
@@ -327,7 +327,7 @@ def main():
         "data": "synthetic (Sod shock tube initial condition)",
         "config": {"workload": f"sod_{N}^3_cfl0.8_{K}steps", "box": N, "nbound": NB, "gamma": "5/3",
                    "path": head},
-        "roofline": roofline_of(r, N),
+        "roofline": roofline_of(r, N, head),
         "step_roofline_frac": round(cups[head] * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9), 4),
         "fp64_cell_updates_per_s": round(cups["fp64"], 1) if "fp64" in cups else None,
         "mixed_cell_updates_per_s": round(cups["mixed"], 1) if "mixed" in cups else None,
@@ -337,7 +337,7 @@ def main():
         "newton_iters_max": {p: v["newton_max"] for p, v in results.items()},
     }
     if "fp64" in results and head != "fp64":
-        line["roofline_fp64"] = roofline_of(results["fp64"], N)
+        line["roofline_fp64"] = roofline_of(results["fp64"], N, "fp64")
     if not args.no_extras:
         log("bench: cosmological 256^3")
         line["cosmo_256"] = run_cosmo(hp, ic)
