@@ -56,9 +56,16 @@ __global__ void __launch_bounds__(256) batch_x(const BatchBox* __restrict__ boxe
     // unit indices are wave-uniform; readfirstlane lets the table search and
     // the descriptor loads run on the scalar unit
     const int u0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    int cur = -1;                                    // box whose tallies t holds
+    Tally t = tally0();
     for (int u = u0; u < total; u += nwaves) {
         const int i = __builtin_amdgcn_readfirstlane(table_find(T, u));
         const int b = __builtin_amdgcn_readfirstlane(T.box[i]);
+        if (b != cur) {                              // one flush per box a wave visits
+            if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);
+            t = tally0();
+            cur = b;
+        }
         const BatchBox& X = boxes[b];
         const int local = u - T.start[i];
         BoxView V;
@@ -68,10 +75,9 @@ __global__ void __launch_bounds__(256) batch_x(const BatchBox* __restrict__ boxe
         const unsigned c0 = (unsigned)local * (unsigned)X.xcpu;
         const unsigned c1 = min(c0 + (unsigned)X.xcpu, nchunks);
         const AllScalars<R> A = load_scalars<M, R>(X.prm);
-        Tally t = tally0();
         x_run<M, R, false, true, EXP>(V, A, E, c0, c1, t);
-        flush_counters(X.st, t, slot == 2);
     }
+    if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);
 }
 
 template <class M, class R, int DIR, bool EXP>
@@ -85,9 +91,16 @@ __global__ void __launch_bounds__(256) batch_stream(const BatchBox* __restrict__
     // unit indices are wave-uniform; readfirstlane lets the table search and
     // the descriptor loads run on the scalar unit
     const int u0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    int cur = -1;
+    Tally t = tally0();
     for (int u = u0; u < total; u += nwaves) {
         const int i = __builtin_amdgcn_readfirstlane(table_find(T, u));
         const int b = __builtin_amdgcn_readfirstlane(T.box[i]);
+        if (b != cur) {
+            if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);
+            t = tally0();
+            cur = b;
+        }
         const BatchBox& X = boxes[b];
         const int local = u - T.start[i];
         BoxView V;
@@ -102,10 +115,9 @@ __global__ void __launch_bounds__(256) batch_stream(const BatchBox* __restrict__
         const int seg = (int)(it / (unsigned)npen);
         const long pen = (long)(it - (unsigned)seg * (unsigned)npen);
         const AllScalars<R> A = load_scalars<M, R>(X.prm);
-        Tally t = tally0();
         stream_run<M, R, DIR, false, true, EXP>(V, A, E, pen, seg, X.seglen[DIR - 2], active, t);
-        flush_counters(X.st, t, slot == 2);
     }
+    if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);
 }
 
 template <class M, class R, bool EXP>

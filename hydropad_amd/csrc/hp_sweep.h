@@ -347,6 +347,46 @@ __device__ __forceinline__ void flush_counters(DevStatus* st, const Tally& T, bo
     }
 }
 
+// Block-level form for the single-box kernels: the waves' partial results
+// meet in LDS and one thread issues the atomics, so a launch of many short
+// waves does not serialise on the status block.  Every wave of the block
+// must call it (no early return before it).
+__device__ __forceinline__ void flush_counters_block(DevStatus* st, const Tally& T, bool want_speed)
+{
+    __shared__ int s_nit[4], s_nc[4], s_cl[4];
+    __shared__ double s_vm[4];
+    int nitmax = T.nitmax, nonconv = T.nonconv, nclamp = T.nclamp;
+    double vmax = T.vmax;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        nitmax = max(nitmax, __shfl_xor(nitmax, off, 64));
+        nonconv += __shfl_xor(nonconv, off, 64);
+        nclamp += __shfl_xor(nclamp, off, 64);
+        if (want_speed) vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_nit[w] = nitmax;
+        s_nc[w] = nonconv;
+        s_cl[w] = nclamp;
+        s_vm[w] = vmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int k = 1; k < nw; ++k) {
+            nitmax = max(nitmax, s_nit[k]);
+            nonconv += s_nc[k];
+            nclamp += s_cl[k];
+            vmax = fmax(vmax, s_vm[k]);
+        }
+        if (nitmax > 0) atomicMax(&st->newton_max, nitmax);
+        if (nonconv) atomicAdd(&st->newton_nonconv, (unsigned long long)nonconv);
+        if (nclamp) atomicAdd(&st->clamps, (unsigned long long)nclamp);
+        if (want_speed) atomicMax(&st->velmax_bits, (unsigned long long)__double_as_longlong(vmax));
+    }
+}
+
 // Newton bookkeeping of one interface.
 __device__ __forceinline__ void count_newton(int nit, int& nitmax, int& nonconv)
 {
@@ -480,12 +520,13 @@ __global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, Epilogue E, int c
     const unsigned nchunks = x_chunks(B.nx, B.ny, B.nz, B.nb);
     const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const unsigned c0 = wave * (unsigned)chunks_per_wave;
-    if (c0 >= nchunks) return;                         // whole wave past the end (uniform)
     const unsigned c1 = min(c0 + (unsigned)chunks_per_wave, nchunks);
-    const AllScalars<R> A = load_scalars<M, R>(B.prm);
     Tally T = tally0();
-    x_run<M, R, GRAV, COSMO, EXP>(B, A, E, c0, c1, T);
-    flush_counters(B.st, T, E.do_speed != 0);
+    if (c0 < c1) {                                     // waves past the end only join the flush
+        const AllScalars<R> A = load_scalars<M, R>(B.prm);
+        x_run<M, R, GRAV, COSMO, EXP>(B, A, E, c0, c1, T);
+    }
+    flush_counters_block(B.st, T, E.do_speed != 0);
 }
 
 // ============================================================================
@@ -621,7 +662,7 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, Epilogue E, int s
     const AllScalars<R> A = load_scalars<M, R>(B.prm);
     Tally T = tally0();
     stream_run<M, R, DIR, GRAV, COSMO, EXP>(B, A, E, pen, seg, seglen, active, T);
-    flush_counters(B.st, T, E.do_speed != 0);
+    flush_counters_block(B.st, T, E.do_speed != 0);
 }
 
 // host launcher of one sweep
@@ -642,7 +683,9 @@ static hipError_t launch_sweep_t(const BoxView& b, const SweepLaunch& L, hipStre
         const long nchunks = (total + 57) / 58;
         // about 16 resident rounds of waves over 256 CUs x 4 SIMDs; each wave
         // walks its run of chunks with a one-chunk register prefetch
-        long cpw = nchunks / 16384;
+        // waves of 16 chunks (fewer for small boxes, to keep ~16K waves):
+        // short runs balance the divergent Riemann work across the chip
+        long cpw = std::min<long>(16, nchunks / 16384);
         if (L.xcpw > 0) cpw = L.xcpw;
         if (cpw < 1) cpw = 1;
         const long waves = (nchunks + cpw - 1) / cpw;
