@@ -427,13 +427,12 @@ __device__ __forceinline__ void newton_side(const RScal& Rs, bool shock, double 
 // guess in pstar; the same iterates as the reference up to rounding.  vstar is
 // the reference's (pl - pr + a vl + b vr)/(a + b) with the last iteration's
 // impedances, written with 1/a and 1/b.
+// cleft/cright are the Lagrangian sound speeds sqrt(gamma p rho) of the two
+// sides (riemann.F90:68-69), rcl/rcr their inverses.
 __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vstar, double pleft, double pright,
-                                      double rholeft, double rhoright, double vxleft, double vxright)
+                                      double vxleft, double vxright, double cleft, double rcl, double cright,
+                                      double rcr)
 {
-    const double gamma = Rs.gamma;
-    double rcl, cleft, rcr, cright;
-    rsq_sqrt(gamma * pleft * rholeft, rcl, cleft);
-    rsq_sqrt(gamma * pright * rhoright, rcr, cright);
     const double cl2 = cleft * cleft, cr2 = cright * cright;
     const double rpl = rcp(pleft), rpr = rcp(pright);
     double ra = 0.0, rb = 0.0;
@@ -485,8 +484,12 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
     const R vxleft = L.u, vxright = Rt.u;
     R pm = R(0), rhom = R(0), vxm = R(0);
     nit = 0;
-    const R cleft = fsqrt(gamma * pleft * rcp(rholeft));
-    const R cright = fsqrt(gamma * pright * rcp(rhoright));
+    // Eulerian sound speeds and their inverses from one rsq each; the
+    // Newton iteration's Lagrangian C = rho c and 1/C = (1/c)(1/rho) follow
+    const double rrl = rcp(rholeft), rrr = rcp(rhoright);
+    double rcle, cleft, rcre, cright;
+    rsq_sqrt(gamma * pleft * rrl, rcle, cleft);
+    rsq_sqrt(gamma * pright * rrr, rcre, cright);
     R p1, p2, rho1, c1, c2;
     if (pleft <= pright) { p1 = pleft; p2 = pright; rho1 = rholeft; c1 = cleft; c2 = cright; }
     else { p1 = pright; p2 = pleft; rho1 = rhoright; c1 = cright; c2 = cleft; }
@@ -509,7 +512,8 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
                 pmd = pow_rm(Rs, div(cleft + cright + (vxleft - vxright) * (gamma - R(1)) * R(0.5),
                                      div(cleft, pow_m(Rs, pleft)) + div(cright, pow_m(Rs, pright))));
         }
-        nit = fm::newton(Rs, pmd, vxmd, pleft, pright, rholeft, rhoright, vxleft, vxright);
+        nit = fm::newton(Rs, pmd, vxmd, pleft, pright, vxleft, vxright, rholeft * cleft, rcle * rrl,
+                         rhoright * cright, rcre * rrr);
         pm = (R)pmd;
         vxm = (R)vxmd;
         if (kase == 1) {
@@ -553,7 +557,7 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
             }
         }
     } else if (kase == 3) {                               // two rarefactions (no Newton)
-        const R zeta = cright * rcp(cleft) * pow_m(Rs, div(pleft, pright));
+        const R zeta = cright * rcle * pow_m(Rs, div(pleft, pright));
         const R vtleft = vxleft + R(2) * cleft * rgamma1;
         const R vtright = vxright - R(2) * cright * rgamma1;
         vxm = div(vtleft * zeta + vtright, R(1) + zeta);
