@@ -431,6 +431,7 @@ int hp_batch_add(hp_batch* b, int nx, int ny, int nz, int nbound, int precision,
 {
     if (!b || !id || !p || !rho || !vx || !vy || !vz || nsteps < 1 || !(cour > 0.0)) return HP_E_ARG;
     if (nbound != 3 || nx < 2 * nbound + 3 || ny < 2 * nbound + 3 || nz < 2 * nbound + 3) return HP_E_ARG;
+    if ((long long)nx * ny * nz >= (1LL << 31)) return HP_E_ARG;   // 32-bit cell indexing in the sweeps
     if (precision != HP_FP64 && precision != HP_MIXED) return HP_E_ARG;
     if (!(gamma > 1.0)) return HP_E_ARG;
     const int slot = find_slot(b);
@@ -454,6 +455,7 @@ int hp_batch_add(hp_batch* b, int nx, int ny, int nz, int nbound, int precision,
 int hp_batch_add_sod(hp_batch* b, int n, int precision, uint64_t seed, double cour, int nsteps, int* id)
 {
     if (!b || !id || n < 3 || nsteps < 1 || !(cour > 0.0)) return HP_E_ARG;
+    if ((long long)(n + 6) * (n + 6) * (n + 6) >= (1LL << 31)) return HP_E_ARG;
     if (precision != HP_FP64 && precision != HP_MIXED) return HP_E_ARG;
     const int nb = 3, na = n + 2 * nb;
     const int slot = find_slot(b);

@@ -209,6 +209,9 @@ int hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out)
     if (nbound != 3) return HP_E_ARG;   // the STENCIL path hard-wires a 6-cell stencil (ppmsolver.F90:82-84)
     if (nx < 2 * nbound + 3 || ny < 2 * nbound + 3 || nz < 2 * nbound + 3) return HP_E_ARG;
     if (precision != HP_FP64 && precision != HP_MIXED && precision != HP_FP64_EXACT) return HP_E_ARG;
+    // the sweeps index cells of one box with 32-bit arithmetic: nx*ny*nz < 2^31
+    // (a 1290^3 box is already past the 288 GB of HBM with two field sets)
+    if ((long long)nx * ny * nz >= (1LL << 31)) return HP_E_ARG;
     hp_ctx* c = new (std::nothrow) hp_ctx();
     if (!c) return HP_E_NOMEM;
     c->nx = nx; c->ny = ny; c->nz = nz; c->nb = nbound;

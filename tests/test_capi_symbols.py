@@ -60,6 +60,7 @@ def test_bad_arguments_rejected_without_gpu():
     h = ctypes.c_void_p()
     assert L.hp_create(70, 70, 70, 2, 0, ctypes.byref(h)) == _capi.HP_E_ARG     # nbound must be 3
     assert L.hp_create(70, 70, 70, 3, 7, ctypes.byref(h)) == _capi.HP_E_ARG     # unknown precision
+    assert L.hp_create(1300, 1300, 1300, 3, 0, ctypes.byref(h)) == _capi.HP_E_ARG   # past 32-bit cell indexing
     assert L.hp_ppmsolver(None, 1) == _capi.HP_E_ARG
 
 
