@@ -1,6 +1,8 @@
 """Small driver for profiler runs (rocprofv3) of the sweep kernels.
 
     python -m hydropad_amd.profiling --size 512 --steps 2 --precision fp64 [--copy-mib 1024]
+    python -m hydropad_amd.profiling --summarize DB [DB ...] [--copy-mib 1024]
+    python -m hydropad_amd.profiling --traffic FETCH_DB WRITE_DB --size 512 --precision fp64 --copy-mib 1024
 
 Uploads the planar Sod shock tube, runs `steps` full steps (x, y, z sweeps,
 speed and timedet on the device) and, with --copy-mib, one plain
@@ -74,6 +76,36 @@ def summarize(paths, copy_bytes=None):
     return "\n".join(out)
 
 
+def traffic_entries(fetch_db, write_db, size, precision, copy_bytes):
+    """HBM bytes per launch of each sweep kernel from one FETCH_SIZE and one
+    WRITE_SIZE run, each counter scaled by what it read for the calibration
+    copy (copy_bytes moved each way by its largest dispatch)."""
+    def load(path, cn):
+        db = sqlite3.connect(path)
+        vals = collections.defaultdict(list)
+        for name, v in db.execute("select kernel_name, value from counters_collection where counter_name=?", (cn,)):
+            vals[name.split("(")[0]].append(v)
+        return vals
+    f, w = load(fetch_db, "FETCH_SIZE"), load(write_db, "WRITE_SIZE")
+    cf = [k for k in f if "copyBuffer" in k][0]
+    cw = [k for k in w if "copyBuffer" in k][0]
+    fscale = copy_bytes / (max(f[cf]) * 1024.0)
+    wscale = copy_bytes / (max(w[cw]) * 1024.0)
+    out = {}
+    alg = 80 * size ** 3
+    for k in f:
+        if "sweep" not in k or k not in w:
+            continue
+        short = "sweep_lane_x" if "lane_x" in k else ("sweep_stream<y>" if ", 2," in k else "sweep_stream<z>")
+        fb = float(np.mean(f[k])) * 1024.0 * fscale
+        wb = float(np.mean(w[k])) * 1024.0 * wscale
+        out[f"{short}@{size}@{precision}"] = {
+            "bytes_per_launch": round(fb + wb), "fetch_bytes": round(fb), "write_bytes": round(wb),
+            "alg_bytes": alg, "ratio_to_alg": round((fb + wb) / alg, 4),
+            "fetch_scale_bytes_per_KiB": fscale, "write_scale_bytes_per_KiB": wscale, "kernel": k}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--size", type=int, default=512)
@@ -83,7 +115,19 @@ def main():
     ap.add_argument("--axis", type=int, default=0, choices=(0, 1, 2), help="shock-tube axis")
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--summarize", nargs="+", metavar="DB", help="print a summary of rocprofv3 databases and exit")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="merge per-launch HBM bytes of the sweep kernels into --traffic-out and exit")
+    ap.add_argument("--traffic-out", default="profiles/traffic.json")
     args = ap.parse_args()
+    if args.traffic:
+        import json
+        import os
+        tr = json.load(open(args.traffic_out)) if os.path.exists(args.traffic_out) else {}
+        tr.update(traffic_entries(args.traffic[0], args.traffic[1], args.size, args.precision,
+                                  args.copy_mib * (1 << 20)))
+        json.dump(tr, open(args.traffic_out, "w"), indent=1)
+        print(json.dumps({k: v["bytes_per_launch"] for k, v in tr.items()}, indent=1))
+        return
     if args.summarize:
         print(summarize(args.summarize, args.copy_mib * (1 << 20) if args.copy_mib else None))
         return
