@@ -51,6 +51,34 @@ def sod(N, axis=0, left=(1.0, 1.0), right=(0.125, 0.1), x0=0.5):
     return periodic_fill(st)
 
 
+def diagonal_shock(N, nb=NB):
+    """The reference's diagonal 2-D shock (hydropad/initialize_hydro.F90:119-159,
+    chooseinit = 4) on one rank: (rho, p) = (1, 1) where the global indices
+    satisfy igridx + igridy < N + 1 (igrid = array index + 1, array indices
+    nb+1..nb+N), (0.125, 0.1) elsewhere, v = 0."""
+    n = N + 2 * nb
+    st = empty_state((n, n, n))
+    i = np.arange(nb + 1, nb + N + 1)                    # 1-based array indices of the interior
+    left = ((i[:, None] + 1) + (i[None, :] + 1)) < N + 1
+    sl = tuple(slice(nb, nb + N) for _ in range(3))
+    st["rho"][sl] = np.where(left, 1.0, 0.125)[:, :, None]
+    st["p"][sl] = np.where(left, 1.0, 0.1)[:, :, None]
+    return periodic_fill(st)
+
+
+def sedov(N, p_blast=100000.0):
+    """The reference's Sedov point blast (hydropad/initialize_hydro.F90:161-170,
+    chooseinit = 5): rho = p = 1, v = 0, p = 1e5 in the first interior cell."""
+    st = uniform(N, rho=1.0, p=1.0)
+    st["p"][NB, NB, NB] = p_blast
+    return periodic_fill(st)
+
+
+def isentropic(N, gamma=5.0 / 3.0):
+    """Uniform isentropic rest state p = rho^gamma (initialize_hydro.F90:172-179)."""
+    return uniform(N, rho=1.0, p=1.0 ** gamma)
+
+
 def uniform(N, rho=1.0, p=1.0, v=(0.0, 0.0, 0.0)):
     n = N + 2 * NB
     st = empty_state((n, n, n))

@@ -313,3 +313,40 @@ def test_near_vacuum_two_rarefactions(oracle, precision):
         assert np.isfinite(out[k]).all(), k
     tol = 1e-5 if precision == "mixed" else TOL_L1
     assert_close(out, ref, tol=tol, what=precision)
+
+
+REF_ICS = [("diagonal_shock", lambda N: ic.diagonal_shock(N)), ("sedov", lambda N: ic.sedov(N)),
+           ("isentropic", lambda N: ic.isentropic(N))]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64_exact", "mixed"])
+@pytest.mark.parametrize("name,make", REF_ICS, ids=[c[0] for c in REF_ICS])
+def test_reference_verification_ics(oracle, name, make, precision):
+    """The reference's own verification set-ups (initialize_hydro.F90
+    chooseinit 4, 5, 6): diagonal 2-D shock, Sedov point blast (p ratio 1e5:
+    strong shocks, long Newton runs), isentropic rest state; 12 steps at 24^3
+    against the oracle, with mass conserved."""
+    N, K = 24, 12
+    print(f"reference IC {name} {precision}", flush=True)
+    st = make(N)
+    P = oracle.params(ngrid=N)
+    ref = copy_state(st)
+    dt, t = oracle.timedet(P, oracle.speed(P, ref), 0.8, 1e30, 0.0)
+    oracle.evolve(P, ref, 1, K, 0.8, 1e30, t)
+    with hp.Box(N, precision) as box:
+        box.upload(copy_state(st))
+        dt_g, t_g = box.timedet(box.speed(), 0.8)
+        box.evolve(1, K, 0.8, t_g, dt_g)
+        out = box.download()
+        c = box.counters()
+    assert c.newton_nonconv == 0
+    tol = 1e-5 if precision == "mixed" else TOL_L1
+    if precision == "mixed":
+        assert rel_l1(out["rho"], ref["rho"]) <= tol
+    else:
+        assert_close(out, ref, tol=tol, what=(name, precision))
+    m0 = st["rho"][NB:-NB, NB:-NB, NB:-NB].sum()
+    assert abs(out["rho"][NB:-NB, NB:-NB, NB:-NB].sum() - m0) <= 1e-12 * m0
+    if name == "isentropic":
+        for k in FIELDS:
+            assert np.array_equal(out[k], st[k]), k
