@@ -16,6 +16,7 @@
 // hydropad/exchange_mesh.F90:47-123) in the same pass.
 #pragma once
 #include <algorithm>
+#include <type_traits>
 #include "hp_core.h"
 #include "hp_fast.h"
 #include "hp_kernels.h"
@@ -433,22 +434,34 @@ __device__ __forceinline__ void x_run(const BoxView& B, const AllScalars<R>& A, 
         if (GRAV) cd[NV - 1] = -B.gforce[3 * q + 0];
     };
 
-    long qn;
-    int in_, jn, kn;
-    bool rn;
+    // The FP64 form prefetches the next chunk into registers; the mixed form
+    // does not: its ten fewer registers give four waves per SIMD instead of
+    // three, which hides the loads better than the prefetch does.
+    constexpr bool PREFETCH = std::is_same<R, double>::value;
+    long qn = 0;
+    int in_ = 0, jn = 0, kn = 0;
+    bool rn = false;
     double nxt[NV];
-    locate(c0, qn, in_, jn, kn, rn);
-    load(qn, nxt);
+    if constexpr (PREFETCH) {
+        locate(c0, qn, in_, jn, kn, rn);
+        load(qn, nxt);
+    }
     for (unsigned c = c0; c < c1; ++c) {
-        const long q = qn;
-        const int i = in_, j = jn, k = kn;
-        const bool in_range = rn;
+        long q;
+        int i, j, k;
+        bool in_range;
         double cd[NV];
+        if constexpr (PREFETCH) {
+            q = qn; i = in_; j = jn; k = kn; in_range = rn;
 #pragma unroll
-        for (int s = 0; s < NV; ++s) cd[s] = nxt[s];
-        if (c + 1 < c1) {                              // prefetch the next chunk
-            locate(c + 1, qn, in_, jn, kn, rn);
-            load(qn, nxt);
+            for (int s = 0; s < NV; ++s) cd[s] = nxt[s];
+            if (c + 1 < c1) {                          // prefetch the next chunk
+                locate(c + 1, qn, in_, jn, kn, rn);
+                load(qn, nxt);
+            }
+        } else {
+            locate(c, q, i, j, k, in_range);
+            load(q, cd);
         }
 
         double am1[NV], ap1[NV], iv[NV], ivr[NV];
