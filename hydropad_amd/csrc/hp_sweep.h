@@ -13,7 +13,10 @@
 // Both read the source buffer and write the other one (ping-pong), so pencils
 // can be split into segments without a read/write race, and they write the
 // periodic face-ghost images of what they store (the one-rank exchange_mesh,
-// hydropad/exchange_mesh.F90:47-123) in the same pass.
+// hydropad/exchange_mesh.F90:47-123) in the same pass.  The last sweep of a
+// step also folds in the CFL maximum (hydropad/speed.F90) and, for comoving
+// boxes, the expansion step (hydropad/expand.F90).  The per-box bodies x_run
+// and stream_run are shared with the persistent batch kernels.
 #pragma once
 #include <algorithm>
 #include <type_traits>
