@@ -21,10 +21,29 @@ __device__ __forceinline__ int table_find(const BatchTable& T, int u)
     return lo;
 }
 
+// v with every 32-bit word taken from the wave's first active lane.  The
+// per-box descriptors and scalars are wave-uniform, but the kernel reads them
+// after its own counter atomics, through pointers it also writes, so the
+// compiler loads them per lane; this returns them to scalar registers.
+template <class T> __device__ __forceinline__ T wave_uniform(const T& v)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    int w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
 __device__ __forceinline__ void batch_view(const BatchBox& X, int slot, BoxView& V, Epilogue& E)
 {
     int in, out, old;
     batch_sets(X.nsets, X.st->nstep_done, slot, in, out, old);
+    in = __builtin_amdgcn_readfirstlane(in);
+    out = __builtin_amdgcn_readfirstlane(out);
+    old = __builtin_amdgcn_readfirstlane(old);
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
         V.in[f] = X.f[in][f];
@@ -41,7 +60,12 @@ __device__ __forceinline__ void batch_view(const BatchBox& X, int slot, BoxView&
     E.old[3] = exp ? X.f[old][4] : nullptr;
     E.fac = X.st->fac;
     E.afact = E.afact1 = E.afact2 = 0.0;
-    if (exp) load_factors(E);
+    if (exp) {
+        load_factors(E);
+        E.afact = wave_uniform(E.afact);
+        E.afact1 = wave_uniform(E.afact1);
+        E.afact2 = wave_uniform(E.afact2);
+    }
     E.ghost_mask = 7;
     E.do_speed = slot == 2;
 }
@@ -74,7 +98,7 @@ __global__ void __launch_bounds__(256) batch_x(const BatchBox* __restrict__ boxe
         const unsigned nchunks = x_chunks(X.nx, X.ny, X.nz, X.nb);
         const unsigned c0 = (unsigned)local * (unsigned)X.xcpu;
         const unsigned c1 = min(c0 + (unsigned)X.xcpu, nchunks);
-        const AllScalars<R> A = load_scalars<M, R>(X.prm);
+        const AllScalars<R> A = wave_uniform(load_scalars<M, R>(X.prm));
         x_run<M, R, false, true, EXP>(V, A, E, c0, c1, t);
     }
     if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);
@@ -114,7 +138,7 @@ __global__ void __launch_bounds__(256) batch_stream(const BatchBox* __restrict__
         const unsigned it = active ? item : 0u;
         const int seg = (int)(it / (unsigned)npen);
         const long pen = (long)(it - (unsigned)seg * (unsigned)npen);
-        const AllScalars<R> A = load_scalars<M, R>(X.prm);
+        const AllScalars<R> A = wave_uniform(load_scalars<M, R>(X.prm));
         stream_run<M, R, DIR, false, true, EXP>(V, A, E, pen, seg, X.seglen[DIR - 2], active, t);
     }
     if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);
