@@ -14,9 +14,9 @@ relative L1/Linf against the FP64 run.
 Prints one JSON line.  `roofline` is the dominant sweep kernel's achieved HBM
 bandwidth (80 algorithmic bytes per cell per sweep / its average HIP-event
 duration) against 8 TB/s; `cpu_baseline` times the oracle's C restatement of
-the reference sweep (bit-identical to the reference's Fortran, one core) on a
-bounded slab of the same workload, with the reference itself timed beside it
-where its build is present.  Beside the headline the line carries the other
+the reference sweep (bit-identical to the reference's Fortran) on bounded
+slabs of the same workload, on the host's cores and on one thread, with the
+reference itself timed beside it where its build is present.  Beside the headline the line carries the other
 workloads of the metric:
   cosmo_256   comoving 256^3 box, a = 1/51 -> 1/11 in 200 steps, FP64 (and mixed)
   ensemble    64 comoving boxes of sizes {32,48,64,96,128,192}^3, own seeds,
@@ -107,13 +107,15 @@ def roofline_of(r, N, prec):
             "alg_bytes_per_launch": alg_bytes, "avg_ms_per_launch": per_dir}
 
 
-def cpu_baseline(N, budget_s=12.0):
-    """CPU baseline on the host cores, one thread.
+def cpu_baseline(N, budget_s=10.0):
+    """CPU baseline on the host cores.
 
-    The timed leg is the oracle's C restatement of the reference's STENCIL
+    The timed legs are the oracle's C restatement of the reference's STENCIL
     sweep ("port": same per-interface structure and arithmetic, bit-identical
-    to the reference) on an N x 32 x 32 slab of the workload (same dx = 1/N,
-    the same shock-tube pencils), run until ~budget_s.  The reference's own
+    to the reference) on slabs of the workload (same dx = 1/N, the same
+    shock-tube pencils), each run until ~budget_s: N x 64 x 64 on all the
+    host's cores (OpenMP over pencils, at most 16 threads -- the CPU share of
+    one GPU on the GPU host), which is `value`, and N x 32 x 32 on one thread.  The reference's own
     Fortran build (oracle/_ref) only runs cubic boxes -- its y/z Riemann
     gathers index rho3d(iaux,j,k) transposed (hydro/ppmsolver.F90:251,379) --
     so it is timed beside the port on one 64^3 step as a calibration."""
@@ -134,19 +136,27 @@ def cpu_baseline(N, budget_s=12.0):
         return st, P
 
     O = Oracle()
-    O.set_threads(1)
-    ntr = 32
-    st, P = shock_tube(N, ntr, N)
-    t0 = time.perf_counter()
-    nsteps = 0
-    while time.perf_counter() - t0 < budget_s and nsteps < 100:
-        nsteps += 1
-        O.ppm_lxlylz(P, st, 1 if nsteps == 1 else nsteps % 6 + 1)
-    el = time.perf_counter() - t0
-    cells = N * ntr * ntr
-    out = {"value": round(cells * nsteps / el, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
-           "sample": f"{N}x{ntr}x{ntr} slab of the {N}^3 shock tube (dx=1/{N}), {nsteps} steps "
-                     f"(3 sweeps + periodic fill each) in {el:.1f} s on one thread"}
+
+    def timed(ntr, threads):
+        O.set_threads(threads)
+        st, P = shock_tube(N, ntr, N)
+        t0 = time.perf_counter()
+        nsteps = 0
+        while time.perf_counter() - t0 < budget_s and nsteps < 100:
+            nsteps += 1
+            O.ppm_lxlylz(P, st, 1 if nsteps == 1 else nsteps % 6 + 1)
+        el = time.perf_counter() - t0
+        O.set_threads(1)
+        return N * ntr * ntr * nsteps / el, nsteps, el
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    v_all, n_all, el_all = timed(64, threads)
+    v_one, n_one, el_one = timed(32, 1)
+    out = {"value": round(v_all, 1), "unit": "cell-updates/s", "cores": threads, "kind": "port",
+           "sample": f"{N}x64x64 slab of the {N}^3 shock tube (dx=1/{N}), {n_all} steps "
+                     f"(3 sweeps + periodic fill each) in {el_all:.1f} s on {threads} threads (OpenMP over pencils)",
+           "single_thread": {"value": round(v_one, 1), "cores": 1,
+                             "sample": f"{N}x32x32 slab, {n_one} steps in {el_one:.1f} s on one thread"}}
     if Reference.available():
         try:
             R = Reference()
