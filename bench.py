@@ -10,6 +10,7 @@ seconds) with the state resident in HBM, and the mixed form's per-field
 relative L1/Linf against the FP64 run.
 
     python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--size 512] [--no-extras]
+                    [--legs cosmo,ensemble,serving]
 
 Prints one JSON line.  `roofline` is the dominant sweep kernel's achieved HBM
 bandwidth (80 algorithmic bytes per cell per sweep / its average HIP-event
@@ -298,7 +299,12 @@ def main():
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--legs", default="cosmo,ensemble,serving",
+                    help="comma-separated other workloads to run beside the headline")
     args = ap.parse_args()
+    legs = set() if args.no_extras else {x.strip() for x in args.legs.split(",") if x.strip()}
+    if legs - {"cosmo", "ensemble", "serving"}:
+        ap.error(f"unknown legs: {sorted(legs - {'cosmo', 'ensemble', 'serving'})}")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note, not a code path)"}))
@@ -348,11 +354,13 @@ def main():
     }
     if "fp64" in results and head != "fp64":
         line["roofline_fp64"] = roofline_of(results["fp64"], N, "fp64")
-    if not args.no_extras:
+    if "cosmo" in legs:
         log("bench: cosmological 256^3")
         line["cosmo_256"] = run_cosmo(hp, ic)
+    if "ensemble" in legs:
         log("bench: ensemble")
         line["ensemble"] = run_ensemble(hp, ic)
+    if "serving" in legs:
         log("bench: serving")
         line["serving"] = run_serving()
     if not args.no_cpu_baseline:
