@@ -1,8 +1,10 @@
 """GPU parity: the HIP path through the C-ABI against the oracle (pinned to
 the reference) and the reference's frozen vectors.
 
-FP64 bar: relative L1 <= 1e-12 per field (in practice the FP64 kernels agree
-bit for bit except where OCML's pow and glibc's differ in the last bit).
+FP64 bar: relative L1 <= 1e-12 per field.  In practice the reference-order
+form (fp64_exact) agrees bit for bit except where OCML's pow and glibc's
+differ in the last bit, and the throughput form (fp64) to about 5e-14 after
+the 64^3 shock tube's 100 steps.
 """
 import numpy as np
 import pytest
