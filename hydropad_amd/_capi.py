@@ -41,7 +41,8 @@ class HpParams(ctypes.Structure):
 
 class HpCounters(ctypes.Structure):
     _fields_ = [("newton_iters_max", ctypes.c_int64), ("newton_nonconv", ctypes.c_int64),
-                ("neg_state_clamps", ctypes.c_int64), ("reserved", ctypes.c_int64 * 5)]
+                ("neg_state_clamps", ctypes.c_int64), ("particles_outside", ctypes.c_int64),
+                ("reserved", ctypes.c_int64 * 4)]
 
 
 COSMO_NAMES = ("cour", "ca", "tstop", "t", "dt", "at", "dat", "ath", "dath", "atnew", "datnew", "rat", "rdtath")
@@ -67,11 +68,12 @@ class HydropadError(RuntimeError):
 EXPORTS = ("hp_params_init", "hp_create", "hp_destroy", "hp_set_precision", "hp_upload",
            "hp_download", "hp_upload_device", "hp_download_device", "hp_set_params",
            "hp_ppmsolver", "hp_exchange_mesh", "hp_ppm_lxlylz", "hp_speed", "hp_evolve",
-           "hp_status", "hp_reset_counters", "hp_sync", "hp_integrate_cell", "hp_dualenergy_cell",
+           "hp_status", "hp_reset_counters", "hp_set_newton_limit", "hp_sync", "hp_integrate_cell", "hp_dualenergy_cell",
            "hp_sweep_timing_enable", "hp_sweep_timing", "hp_set_tiling", "hp_cosmo_init", "hp_timedet_cosmo",
            "hp_cosmo_params", "hp_savetn", "hp_expand", "hp_evolve_cosmo",
+           "hp_particles_upload", "hp_particles_download", "hp_kick", "hp_drift",
            "hp_batch_box_bytes", "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_add_sod",
-           "hp_sod_of_seed", "hp_batch_remove", "hp_batch_download", "hp_batch_step", "hp_batch_sync",
+           "hp_sod_of_seed", "hp_batch_remove", "hp_batch_set_newton_limit", "hp_batch_download", "hp_batch_step", "hp_batch_sync",
            "hp_batch_poll", "hp_batch_use_graph", "hp_batch_set_tuning", "hp_batch_get_tuning",
            "hp_batch_bytes_used", "hp_batch_autotune", "hp_serve_run")
 
@@ -104,6 +106,7 @@ def lib():
     L.hp_evolve.argtypes = [_P, _I, _I, _D, _D, _PD, _PD]
     L.hp_status.argtypes = [_P, ctypes.POINTER(HpCounters)]
     L.hp_reset_counters.argtypes = [_P]
+    L.hp_set_newton_limit.argtypes = [_P, _I]
     L.hp_sync.argtypes = [_P]
     L.hp_integrate_cell.argtypes = [PP] + [_PD] * 18
     L.hp_dualenergy_cell.argtypes = [PP] + [_D] * 8 + [_PD, _PD]
@@ -117,6 +120,10 @@ def lib():
     L.hp_savetn.argtypes = [_P]
     L.hp_expand.argtypes = [_P, _D]
     L.hp_evolve_cosmo.argtypes = [_P, _I, _I, PC]
+    L.hp_particles_upload.argtypes = [_P, ctypes.c_int64, _P, _P, _D]
+    L.hp_particles_download.argtypes = [_P, _P, _P]
+    L.hp_kick.argtypes = [_P, _D]
+    L.hp_drift.argtypes = [_P]
     _lib = L
     return L
 

@@ -37,7 +37,7 @@ class HpBoxStatus(ctypes.Structure):
 
 class HpRequest(ctypes.Structure):
     _fields_ = [("id", _I64), ("n", ctypes.c_int32), ("nsteps", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("seed", _U64), ("arrival_s", _D), ("cour", _D)]
+                ("newton_limit", ctypes.c_int32), ("seed", _U64), ("arrival_s", _D), ("cour", _D)]
 
 
 class HpRequestResult(ctypes.Structure):
@@ -71,6 +71,7 @@ def _lib():
         L.hp_batch_add_sod.argtypes = [PB, _I, _I, _U64, _D, _I, ctypes.POINTER(_I)]
         L.hp_sod_of_seed.argtypes = [_U64, ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)]
         L.hp_batch_remove.argtypes = [PB, _I]
+        L.hp_batch_set_newton_limit.argtypes = [PB, _I, _I]
         L.hp_batch_download.argtypes = [PB, _I] + [_P] * 5
         L.hp_batch_step.argtypes = [PB, _I]
         L.hp_batch_sync.argtypes = [PB]
@@ -148,6 +149,9 @@ class Batch:
         check("hp_batch_remove", self.L.hp_batch_remove(self._h, int(i)))
         self.shapes.pop(i, None)
 
+    def set_newton_limit(self, i, nmax=0):
+        check("hp_batch_set_newton_limit", self.L.hp_batch_set_newton_limit(self._h, int(i), int(nmax)))
+
     def download(self, i):
         out = {k: np.zeros(self.shapes[i], dtype=np.float64, order="F") for k in FIELDS}
         check("hp_batch_download", self.L.hp_batch_download(self._h, int(i),
@@ -197,15 +201,17 @@ class Batch:
 
 def serve(requests, hbm_budget=64 << 30, max_boxes=64, use_graph=True, keep_fields=()):
     """Run the request loop.  requests: list of dicts with n, nsteps,
-    precision ("fp64"/"mixed"), seed, arrival_s (and optional id, cour).
+    precision ("fp64"/"mixed"), seed, arrival_s (and optional id, cour,
+    newton_limit).
     keep_fields: indices of requests whose final states are returned.
     Returns (results list of dicts, stats dict, {index: state})."""
     L = _lib()
     nreq = len(requests)
     arr = (HpRequest * max(nreq, 1))()
     for i, r in enumerate(requests):
-        arr[i] = HpRequest(int(r.get("id", i)), int(r["n"]), int(r["nsteps"]), PRECISIONS[r["precision"]], 0,
-                           int(r["seed"]), float(r["arrival_s"]), float(r.get("cour", 0.8)))
+        arr[i] = HpRequest(int(r.get("id", i)), int(r["n"]), int(r["nsteps"]), PRECISIONS[r["precision"]],
+                           int(r.get("newton_limit", 0)), int(r["seed"]), float(r["arrival_s"]),
+                           float(r.get("cour", 0.8)))
     res = (HpRequestResult * max(nreq, 1))()
     stats = HpServeStats()
     fields = {}

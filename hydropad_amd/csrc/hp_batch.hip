@@ -569,6 +569,14 @@ int hp_batch_step(hp_batch* b, int nsteps)
     return HP_OK;
 }
 
+int hp_batch_set_newton_limit(hp_batch* b, int id, int nmax)
+{
+    if (!b || id < 0 || id >= b->maxb || !b->h_boxes[id].active || nmax < 0) return HP_E_ARG;
+    HPCHK(hipMemcpyAsync(&b->h_boxes[id].st->newton_cap, &nmax, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HPCHK(hipStreamSynchronize(b->stream));
+    return HP_OK;
+}
+
 int hp_batch_sync(hp_batch* b)
 {
     if (!b) return HP_E_ARG;

@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(256) batch_x(const BatchBox* __restrict__ boxe
         const unsigned nchunks = x_chunks(X.nx, X.ny, X.nz, X.nb);
         const unsigned c0 = (unsigned)local * (unsigned)X.xcpu;
         const unsigned c1 = min(c0 + (unsigned)X.xcpu, nchunks);
-        const AllScalars<R> A = wave_uniform(load_scalars<M, R>(X.prm));
+        const AllScalars<R> A = wave_uniform(load_scalars<M, R>(X.prm, X.st));
         x_run<M, R, false, true, EXP>(V, A, E, c0, c1, t);
     }
     if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(256) batch_stream(const BatchBox* __restrict__
         const unsigned it = active ? item : 0u;
         const int seg = (int)(it / (unsigned)npen);
         const long pen = (long)(it - (unsigned)seg * (unsigned)npen);
-        const AllScalars<R> A = wave_uniform(load_scalars<M, R>(X.prm));
+        const AllScalars<R> A = wave_uniform(load_scalars<M, R>(X.prm, X.st));
         stream_run<M, R, DIR, false, true, EXP>(V, A, E, pen, seg, X.seglen[DIR - 2], active, t);
     }
     if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);

@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -41,9 +42,9 @@ __global__ void integrate_cells_kernel(hp::IScal S, int n, const double* __restr
     F1.p = f[0]; F2.p = f[1]; F1.r = f[2]; F2.r = f[3]; F1.u = f[4]; F2.u = f[5];
     F1.v = f[6]; F2.v = f[7]; F1.w = f[8]; F2.w = f[9];
     double pres = s[0], rho = s[1], vx = s[2], vy = s[3], vz = s[4];
-    // integrate() takes etot/eint from its caller (ppmsolver.F90:149-150);
-    // the cell function derives eint = pres*rgamma1 itself, as internal() does.
-    hp::integrate_cell(S, F1, F2, pres, rho, vx, vy, vz, e[0], e[2], e[2] != 0.0);
+    // integrate() takes etot and eint from its caller (integrate.F90:1-2) and
+    // always adds the gravity terms, zero when ghalf is (integrate.F90:43-44,56-57)
+    hp::integrate_cell_e(S, F1, F2, pres, rho, vx, vy, vz, e[0], e[1], e[2], true);
     s[0] = pres; s[1] = rho; s[2] = vx; s[3] = vy; s[4] = vz;
 }
 
@@ -86,6 +87,10 @@ struct hp_ctx {
     int xcpw;
     int block;
     bool timing;
+    double* ppos;          // particles (3,np): positions, velocities (the reference's ppos/pvel)
+    double* pvel;
+    long long np, pcap;
+    double dtat;           // the kicks' time step (the reference's module scalar dtat)
     std::vector<hipEvent_t> ev_pool;     // recycled events
     std::vector<hipEvent_t> ev_used[3];  // start/end pairs per direction
 };
@@ -174,6 +179,28 @@ static int sweep(hp_ctx* c, int dir, int ghost_mask, int do_speed, bool expand =
     return HP_OK;
 }
 
+// The reference stops the run when a Riemann Newton iteration does not
+// converge (riemann.F90:60-65); here the device counter, cumulative since the
+// last hp_reset_counters, turns into HP_E_NEWTON at the call's return.
+static int newton_rc(hp_ctx* c)
+{
+    unsigned long long n = 0;
+    HPCHK(hipMemcpyAsync(&n, &c->d_st->newton_nonconv, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+    HPCHK(hipStreamSynchronize(c->stream));
+    return n ? HP_E_NEWTON : HP_OK;
+}
+
+// The N-body part of a step (hydropad/evolve_sys.F90:63-84): kick, drift,
+// kick over the particles with the context's gforce, their speeds folded into
+// the CFL maximum (speed.F90:43-56).
+static int particle_step(hp_ctx* c)
+{
+    if (c->np <= 0) return HP_OK;
+    HPCHK(hp::launch_particles(3, c->ppos, c->pvel, c->np, c->has_grav ? c->gforce : nullptr, c->nx, c->ny, c->nz,
+                               c->nb, c->d_prm, c->dtat, c->d_st, 1, c->stream));
+    return HP_OK;
+}
+
 extern "C" {
 
 int hp_params_init(double gamma, int ngrid_max, double dt, hp_params* o)
@@ -221,6 +248,9 @@ int hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out)
     c->xcpw = 0;
     c->block = 256;
     c->timing = false;
+    c->ppos = c->pvel = nullptr;
+    c->np = c->pcap = 0;
+    c->dtat = 0.0;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HP_E_HIP; }
     const size_t fbytes = c->ncell * sizeof(double);
     const size_t align = 256;
@@ -239,7 +269,8 @@ int hp_create(int nx, int ny, int nz, int nbound, int precision, hp_ctx** out)
     for (int f = 0; f < 5; ++f) c->f[2][f] = nullptr;
     c->has_grav = false;
     if (hipStreamCreate(&c->stream) != hipSuccess) { hipFree(c->pool); delete c; return HP_E_HIP; }
-    hp_params_init(5.0 / 3.0, nx - 2 * nbound, 0.0, &c->h_prm);
+    // dx from the largest interior extent (initialize_parameters.F90:23)
+    hp_params_init(5.0 / 3.0, std::max(nx, std::max(ny, nz)) - 2 * nbound, 0.0, &c->h_prm);
     hp::DevStatus z;
     memset(&z, 0, sizeof(z));
     if (hipMemcpyAsync(c->d_prm, &c->h_prm, sizeof(hp_params), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
@@ -261,6 +292,7 @@ void hp_destroy(hp_ctx* c)
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->pool) hipFree(c->pool);
     if (c->pool3) hipFree(c->pool3);
+    if (c->ppos) hipFree(c->ppos);
     delete c;
 }
 
@@ -337,7 +369,8 @@ int hp_set_params(hp_ctx* c, const hp_params* prm)
 int hp_ppmsolver(hp_ctx* c, int ndirection)
 {
     if (!c || ndirection < 1 || ndirection > 3) return HP_E_ARG;
-    return sweep(c, ndirection, 7, 0);
+    const int r = sweep(c, ndirection, 7, 0);
+    return r ? r : newton_rc(c);
 }
 
 int hp_exchange_mesh(hp_ctx* c, int direction)
@@ -354,7 +387,7 @@ int hp_ppm_lxlylz(hp_ctx* c, int nrot)
         int r = sweep(c, NROT_ORDER[nrot - 1][s], 7, 0);
         if (r) return r;
     }
-    return HP_OK;
+    return newton_rc(c);
 }
 
 int hp_speed(hp_ctx* c, double* velmax)
@@ -363,6 +396,7 @@ int hp_speed(hp_ctx* c, double* velmax)
     HPCHK(hp::launch_zero_status(c->d_st, 1, c->stream));
     const double* f[5] = {c->f[c->cur][0], c->f[c->cur][1], c->f[c->cur][2], c->f[c->cur][3], c->f[c->cur][4]};
     HPCHK(hp::launch_speed(f, c->nx, c->ny, c->nz, c->nb, c->d_prm, c->d_st, c->stream));
+    HPCHK(hp::launch_particle_vmax(c->pvel, c->np, c->d_st, c->stream));   // speed.F90:43-56
     unsigned long long bits = 0;
     HPCHK(hipMemcpyAsync(&bits, &c->d_st->velmax_bits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
     HPCHK(hp::launch_zero_status(c->d_st, 1, c->stream));
@@ -387,6 +421,7 @@ int hp_evolve(hp_ctx* c, int first_step, int nsteps, double cour, double tstop, 
             int r = sweep(c, NROT_ORDER[nrot - 1][s], 7, s == 2 ? 1 : 0);
             if (r) return r;
         }
+        if (int r = particle_step(c)) return r;
         HPCHK(hp::launch_timedet(c->d_prm, c->d_st, cour, tstop, c->stream));
     }
     HPCHK(hipMemcpyAsync(&c->h_prm, c->d_prm, sizeof(hp_params), hipMemcpyDeviceToHost, c->stream));
@@ -394,7 +429,7 @@ int hp_evolve(hp_ctx* c, int first_step, int nsteps, double cour, double tstop, 
     HPCHK(hipStreamSynchronize(c->stream));
     *t = tt;
     *dt = c->h_prm.dt;
-    return HP_OK;
+    return newton_rc(c);
 }
 
 // ---- comoving boxes ------------------------------------------------------------
@@ -467,13 +502,14 @@ int hp_evolve_cosmo(hp_ctx* c, int first_step, int nsteps, hp_cosmo* s)
             int r = sweep(c, NROT_ORDER[nrot - 1][k], 7, k == 2 ? 1 : 0, k == 2);
             if (r) return r;
         }
+        if (int r = particle_step(c)) return r;
         HPCHK(hp::launch_timedet_cosmo(c->d_prm, c->d_st, c->stream));
     }
     HPCHK(hipMemcpyAsync(&c->h_prm, c->d_prm, sizeof(hp_params), hipMemcpyDeviceToHost, c->stream));
     HPCHK(hipMemcpyAsync(&cs, &c->d_st->cosmo, sizeof(cs), hipMemcpyDeviceToHost, c->stream));
     HPCHK(hipStreamSynchronize(c->stream));
     memcpy(s, &cs, sizeof(cs));
-    return HP_OK;
+    return newton_rc(c);
 }
 
 int hp_set_tiling(hp_ctx* c, int x_chunks, int yz_seglen, int block)
@@ -525,7 +561,73 @@ int hp_status(hp_ctx* c, hp_counters* out)
     out->newton_iters_max = s.newton_max;
     out->newton_nonconv = (int64_t)s.newton_nonconv;
     out->neg_state_clamps = (int64_t)s.clamps;
+    out->particles_outside = (int64_t)s.outside;
     return s.newton_nonconv ? HP_E_NEWTON : HP_OK;
+}
+
+// ---- particles (nbody/) ------------------------------------------------------
+
+int hp_particles_upload(hp_ctx* c, int64_t np, const double* ppos, const double* pvel, double dtat)
+{
+    if (!c || np < 0 || (np > 0 && (!ppos || !pvel))) return HP_E_ARG;
+    if (np >= (1LL << 40)) return HP_E_ARG;
+    HPCHK(hipStreamSynchronize(c->stream));
+    if (np > c->pcap) {
+        if (c->ppos) hipFree(c->ppos);
+        c->ppos = c->pvel = nullptr;
+        c->np = c->pcap = 0;
+        const size_t half = ((size_t)np * 3 * sizeof(double) + 255) / 256 * 256;
+        void* m = nullptr;
+        if (hipMalloc(&m, 2 * half) != hipSuccess) return HP_E_NOMEM;
+        c->ppos = (double*)m;
+        c->pvel = (double*)((char*)m + half);
+        c->pcap = np;
+    }
+    const size_t bytes = (size_t)np * 3 * sizeof(double);
+    if (np > 0) {
+        HPCHK(hipMemcpyAsync(c->ppos, ppos, bytes, hipMemcpyHostToDevice, c->stream));
+        HPCHK(hipMemcpyAsync(c->pvel, pvel, bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    c->np = np;
+    c->dtat = dtat;
+    HPCHK(hipStreamSynchronize(c->stream));
+    return HP_OK;
+}
+
+int hp_particles_download(hp_ctx* c, double* ppos, double* pvel)
+{
+    if (!c || (c->np > 0 && (!ppos || !pvel))) return HP_E_ARG;
+    const size_t bytes = (size_t)c->np * 3 * sizeof(double);
+    if (c->np > 0) {
+        HPCHK(hipMemcpyAsync(ppos, c->ppos, bytes, hipMemcpyDeviceToHost, c->stream));
+        HPCHK(hipMemcpyAsync(pvel, c->pvel, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+    HPCHK(hipStreamSynchronize(c->stream));
+    return HP_OK;
+}
+
+int hp_kick(hp_ctx* c, double dtat)
+{
+    if (!c) return HP_E_ARG;
+    HPCHK(hp::launch_particles(1, c->ppos, c->pvel, c->np, c->has_grav ? c->gforce : nullptr, c->nx, c->ny, c->nz,
+                               c->nb, c->d_prm, dtat, c->d_st, 0, c->stream));
+    return HP_OK;
+}
+
+int hp_drift(hp_ctx* c)
+{
+    if (!c) return HP_E_ARG;
+    HPCHK(hp::launch_particles(2, c->ppos, c->pvel, c->np, nullptr, c->nx, c->ny, c->nz, c->nb, c->d_prm, 0.0, c->d_st,
+                               0, c->stream));
+    return HP_OK;
+}
+
+int hp_set_newton_limit(hp_ctx* c, int nmax)
+{
+    if (!c || nmax < 0) return HP_E_ARG;
+    HPCHK(hipMemcpyAsync(&c->d_st->newton_cap, &nmax, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HPCHK(hipStreamSynchronize(c->stream));
+    return HP_OK;
 }
 
 int hp_reset_counters(hp_ctx* c)
@@ -554,7 +656,6 @@ int hp_integrate_cell(const hp_params* prm, const double* pm1, const double* pm2
         return HP_E_ARG;
     double h[18] = {*pm1, *pm2, *rhom1, *rhom2, *vxm1, *vxm2, *vym1, *vym2, *vzm1, *vzm2,
                     *pres, *rho, *vx, *vy, *vz, *etot, *eint, *ghalf};
-    (void)eint;   // integrate() recomputes nothing from eint the cell function does not derive itself
     double* d = nullptr;
     if (hipMalloc(&d, sizeof(h)) != hipSuccess) return HP_E_NOMEM;
     int rc = HP_OK;

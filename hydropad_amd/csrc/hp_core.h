@@ -266,6 +266,7 @@ __device__ __forceinline__ Face<R> trace_left_face(const SweepScalars<R>& S, con
 struct RScal {
     double gamma, gamma1, rgamma, rgamma1, m, rm, gf, dadt, dmax;
     int m5;   // m == 1/5 (gamma = 5/3): powers of m, 1/gamma, 1/m are roots/products
+    int nmax; // iterations after which the reference stops (nnewtmax = 100, riemann.F90:51)
 };
 
 __device__ __forceinline__ double rarwave(double f1, double f2, double vt, double vh)
@@ -294,7 +295,7 @@ __device__ __forceinline__ int newton(const RScal& R, double& pstar, double& vst
     const double cright = sqrt(gamma * pright * rhoright);
     while (err > K_TERR) {
         nnewt = nnewt + 1;
-        if (nnewt > 100) return -1;
+        if (nnewt > R.nmax) return -1;
         double pstarold = pstar;
         double pstup = (1. - pow(pstar / pleft, m));
         if (pstar >= pleft) a = cleft * sqrt(gf * pstar / pleft + m);
@@ -481,12 +482,13 @@ __host__ __device__ __forceinline__ void dual_energy(const IScal& S, double etot
     if (eintnew <= 0.0) { eintnew = eint; pnew = pres; }
 }
 
-__device__ __forceinline__ void integrate_cell(const IScal& S, const Flux& F1, const Flux& F2, double& pres,
-                                               double& rho, double& vx, double& vy, double& vz,
-                                               double etot, double ghalf, bool grav)
+// integrate() with the caller's total and internal energy (integrate.F90:1-2:
+// ppmsolver passes etot = total(...) and eint = internal(p), :149-150).
+__device__ __forceinline__ void integrate_cell_e(const IScal& S, const Flux& F1, const Flux& F2, double& pres,
+                                                 double& rho, double& vx, double& vy, double& vz,
+                                                 double etot, double eint, double ghalf, bool grav)
 {
     const double rdtath = S.rdtath, rdx = S.rdx, gamma = S.gamma, rgamma1 = S.rgamma1;
-    const double eint = pres * rgamma1;                       // internal.F90:14
     double flux1, flux2;
     flux1 = F1.r * F1.u;
     flux2 = F2.r * F2.u;
@@ -529,6 +531,14 @@ __device__ __forceinline__ void integrate_cell(const IScal& S, const Flux& F1, c
     vx = vxnew;
     vy = vynew;
     vz = vznew;
+}
+
+// The sweep's form: eint = internal(p) = p/(gamma-1) (internal.F90:14).
+__device__ __forceinline__ void integrate_cell(const IScal& S, const Flux& F1, const Flux& F2, double& pres,
+                                               double& rho, double& vx, double& vy, double& vz,
+                                               double etot, double ghalf, bool grav)
+{
+    integrate_cell_e(S, F1, F2, pres, rho, vx, vy, vz, etot, pres * S.rgamma1, ghalf, grav);
 }
 
 }  // namespace hp

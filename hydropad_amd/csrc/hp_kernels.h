@@ -23,9 +23,10 @@ struct Cosmo {
 struct DevStatus {
     unsigned long long velmax_bits;   // max(|v|+cs) as the bit pattern of a non-negative double
     int newton_max;
-    int pad0;
+    int newton_cap;                    // Newton iteration limit; 0 = the reference's nnewtmax = 100 (riemann.F90:51)
     unsigned long long newton_nonconv;
     unsigned long long clamps;
+    unsigned long long outside;        // particle kicks skipped: CIC cube outside the array
     double t;                          // time at the end of the upcoming step (timedet semantics)
     double fac[3];                     // expansion factors of the upcoming step (expand.F90:18-20)
     Cosmo cosmo;                       // schedule of a cosmological box
@@ -90,5 +91,14 @@ hipError_t launch_expand(double* const f[5], const double* const old[4], long nc
 hipError_t launch_copy(double* const* dst, const double* const* src, int nfield, long ncell, hipStream_t s);
 // what: bit 0 clears the CFL maximum, bit 1 the counters.
 hipError_t launch_zero_status(DevStatus* st, int what, hipStream_t s);
+
+// N-body (hp_nbody.hip): what = 1 kick (nbody/kick.F90), 2 drift
+// (nbody/drift.F90), 3 kick + drift + kick (one step of
+// hydropad/evolve_sys.F90:63-84).  pos/vel are (3,np); gforce may be nullptr
+// (zero force).  speed != 0 folds max |v| into st->velmax_bits
+// (hydropad/speed.F90:43-56); skipped kicks count in st->outside.
+hipError_t launch_particles(int what, double* pos, double* vel, long np, const double* gforce, int nx, int ny, int nz,
+                            int nb, const Params* prm, double dtat, DevStatus* st, int speed, hipStream_t s);
+hipError_t launch_particle_vmax(const double* vel, long np, DevStatus* st, hipStream_t s);
 
 }  // namespace hp

@@ -133,6 +133,7 @@ __global__ void zero_status_kernel(DevStatus* st, int what)
             st->newton_max = 0;
             st->newton_nonconv = 0ull;
             st->clamps = 0ull;
+            st->outside = 0ull;
         }
     }
 }

@@ -35,7 +35,7 @@ extern "C" int hp_serve_run(const hp_request* reqs, int nreq, size_t hbm_budget,
     if (!reqs || nreq < 0 || !results || max_boxes < 1) return HP_E_ARG;
     for (int i = 0; i < nreq; ++i)
         if (reqs[i].n < 3 || reqs[i].nsteps < 1 || (reqs[i].precision != HP_FP64 && reqs[i].precision != HP_MIXED) ||
-            !(reqs[i].cour > 0.0))
+            !(reqs[i].cour > 0.0) || reqs[i].newton_limit < 0)
             return HP_E_ARG;
     hp_batch* b = nullptr;
     int rc = hp_batch_create(max_boxes, hbm_budget, &b);
@@ -83,6 +83,7 @@ extern "C" int hp_serve_run(const hp_request* reqs, int nreq, size_t hbm_budget,
             }
             if (a == HP_E_NOMEM) break;                                // wait for memory / a slot
             if (a != HP_OK) { rc = a; break; }
+            if (r.newton_limit > 0 && (rc = hp_batch_set_newton_limit(b, id, r.newton_limit)) != HP_OK) break;
             results[i].admit_s = now();
             owner[id] = i;
             ++nactive;
@@ -112,6 +113,7 @@ extern "C" int hp_serve_run(const hp_request* reqs, int nreq, size_t hbm_budget,
             results[i].steps = st[id].steps_done;
             results[i].t = st[id].t;
             results[i].newton_nonconv = st[id].newton_nonconv;
+            if (st[id].newton_nonconv) results[i].rc = HP_E_NEWTON;   // the reference stops (riemann.F90:60-65)
             if (result_fields && result_fields[5 * i]) {
                 rc = hp_batch_download(b, id, result_fields[5 * i], result_fields[5 * i + 1], result_fields[5 * i + 2],
                                        result_fields[5 * i + 3], result_fields[5 * i + 4]);

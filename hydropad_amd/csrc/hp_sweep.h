@@ -47,7 +47,7 @@ struct ExactMath {
     static __device__ __forceinline__ void cell_faces(const AllScalars<R>& A, const double* a, const double* ivl,
                                                       const double* ivr, double rm2, double rm1, double rp1,
                                                       double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
-                                                      Face<double>& fL, Face<double>& fR, int& nclamp);
+                                                      Face<double>& fL, Face<double>& fR, int& nclL, int& nclR);
     static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
                                                    int& nit)
     {
@@ -78,7 +78,7 @@ struct FastMath {
     static __device__ __forceinline__ void cell_faces(const AllScalars<R>& A, const double* a, const double* ivl,
                                                       const double* ivr, double rm2, double rm1, double rp1,
                                                       double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
-                                                      Face<double>& fL, Face<double>& fR, int& nclamp);
+                                                      Face<double>& fL, Face<double>& fR, int& nclL, int& nclR);
     static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
                                                    int& nit)
     {
@@ -140,6 +140,7 @@ template <int CTRL> __device__ __forceinline__ float dpp(float x)
 {
     return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
 }
+template <int CTRL> __device__ __forceinline__ int dpp(int x) { return dpp_i<CTRL>(x); }
 template <class T> __device__ __forceinline__ T from_left(T x) { return dpp<DPP_WAVE_SHR1>(x); }
 template <class T> __device__ __forceinline__ T from_right(T x) { return dpp<DPP_WAVE_SHL1>(x); }
 
@@ -178,9 +179,11 @@ struct Gamma53 {
     static constexpr double gf = (gamma + 1.0) / (2.0 * gamma);
 };
 
-template <class M, class R> __device__ __forceinline__ AllScalars<R> load_scalars(const Params* __restrict__ prm)
+template <class M, class R>
+__device__ __forceinline__ AllScalars<R> load_scalars(const Params* __restrict__ prm, const DevStatus* st)
 {
     const Params P = *prm;
+    const int cap = st->newton_cap;
     double gamma = P.gamma, gamma1 = P.gamma1, rgamma = P.rgamma, rgamma1 = P.rgamma1;
     double m = P.m, rm = P.rm, gf = P.gf;
     int m5 = fabs(P.m - 0.2) < 1e-15;
@@ -200,6 +203,7 @@ template <class M, class R> __device__ __forceinline__ AllScalars<R> load_scalar
     A.Rs.dadt = 1.0 - 0.5 * P.dat * P.dt * P.rat;     // riemannsolver.F90:36
     A.Rs.dmax = P.dmax;
     A.Rs.m5 = m5;
+    A.Rs.nmax = cap > 0 ? cap : 100;
     A.Is.rdtath = P.rdtath; A.Is.rdx = P.rdx; A.Is.gamma = gamma; A.Is.rgamma1 = rgamma1;
     A.Is.eta1 = P.eta1; A.Is.eta2 = P.eta2;
     A.rgamma1 = rgamma1; A.gamma = gamma; A.mingradflat = P.mingradflat;
@@ -215,7 +219,7 @@ template <class R, int NV, bool GRAV, bool COSMO>
 __device__ __forceinline__ void ExactMath::cell_faces(const AllScalars<R>& A, const double* a, const double* ivl,
                                                       const double* ivr, double rm2, double rm1, double rp1,
                                                       double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
-                                                      Face<double>& fL, Face<double>& fR, int& nclamp)
+                                                      Face<double>& fL, Face<double>& fR, int& nclL, int& nclR)
 {
     const SweepScalars<double>& S = A.S;
     const double flatv = flat ? S.flatvalue : 0.0;
@@ -229,18 +233,18 @@ __device__ __forceinline__ void ExactMath::cell_faces(const AllScalars<R>& A, co
         q[v] = limit_parab(a[v], ajl, ajr, flatv);
     }
     const double cs = sqrt(S.gamma * a[VP] / a[VR]);                               // ppmsolver.F90:109
-    fR = trace_right_face<double, GRAV, COSMO>(S, q, a[VU], cs, nclamp);
-    fL = trace_left_face<double, GRAV, COSMO>(S, q, a[VU], cs, nclamp);
+    fR = trace_right_face<double, GRAV, COSMO>(S, q, a[VU], cs, nclR);
+    fL = trace_left_face<double, GRAV, COSMO>(S, q, a[VU], cs, nclL);
 }
 
 template <class R, int NV, bool GRAV, bool COSMO>
 __device__ __forceinline__ void FastMath::cell_faces(const AllScalars<R>& A, const double* a, const double* ivl,
                                                      const double* ivr, double rm2, double rm1, double rp1,
                                                      double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
-                                                     Face<double>& fL, Face<double>& fR, int& nclamp)
+                                                     Face<double>& fL, Face<double>& fR, int& nclL, int& nclR)
 {
     fm::cell_faces_d<R, NV, GRAV, COSMO>(A.S, A.gamma, a, ivl, ivr, rm2, rm1, rp1, rp2, drm1, drp1, pm1, pp1, flat,
-                                         fL, fR, nclamp);
+                                         fL, fR, nclL, nclR);
 }
 
 // Store one updated cell and, for the axes in `mask`, its periodic face-ghost
@@ -495,10 +499,10 @@ __device__ __forceinline__ void x_run(const BoxView& B, const AllScalars<R>& A, 
         const double rm2 = from_left(am1[VR]);
         const double rp2 = from_right(ap1[VR]);
         const R drp1 = from_right(d[VR]);
-        int ncl = 0;
+        int nclL = 0, nclR = 0;
         Face<double> fL, fR;
         M::template cell_faces<R, NV, GRAV, COSMO>(A, cd, iv, ivr, rm2, am1[VR], ap1[VR], rp2, dm1[VR], drp1,
-                                                   am1[VP], ap1[VP], flat, fL, fR, ncl);
+                                                   am1[VP], ap1[VP], flat, fL, fR, nclL, nclR);
 
         // Riemann problem at interface c-1/2: right face of c-1 | left face of c
         const Face<double> fRm1 = from_left(fR);
@@ -514,11 +518,20 @@ __device__ __forceinline__ void x_run(const BoxView& B, const AllScalars<R>& A, 
             if ((Mc >> lane) & 1ull) peak_floor(A.Rs, F, r1, rm2, am1[VR], cd[VR], u2, u3);
         }
         const Flux F2 = from_right(F);                                    // interface c+1/2
+        // counters per interface, as the reference's loop over interfaces
+        // nb-1/2 .. nb+N-1/2 (ppmsolver.F90:98-138): interface c-1/2 takes the
+        // clamps of c-1's right face and c's left face; the last interior
+        // cell also counts interface c+1/2, solved by the lane to its right
+        const int nclRm1 = from_left(nclR), nclLp1 = from_right(nclL), nitp1 = from_right(nit);
 
         const bool valid = in_range && lane >= 3 && lane <= 60 && i >= nb && i < nb + Nx;
         if (valid) {
             count_newton(nit, T.nitmax, T.nonconv);
-            T.nclamp += ncl;
+            T.nclamp += nclRm1 + nclL;
+            if (i == nb + Nx - 1) {
+                count_newton(nitp1, T.nitmax, T.nonconv);
+                T.nclamp += nclR + nclLp1;
+            }
             double pres = cd[VP], rho = cd[VR], u = cd[VU], vv = cd[VV], w = cd[VW];
             const double etot = total_phys<1>(A.rgamma1, pres, rho, u, vv, w);
             M::integrate(A.Is, F, F2, pres, rho, u, vv, w, etot, GRAV ? cd[NV - 1] : 0.0, GRAV);
@@ -539,7 +552,7 @@ __global__ void __launch_bounds__(256) sweep_lane_x(BoxView B, Epilogue E, int c
     const unsigned c1 = min(c0 + (unsigned)chunks_per_wave, nchunks);
     Tally T = tally0();
     if (c0 < c1) {                                     // waves past the end only join the flush
-        const AllScalars<R> A = load_scalars<M, R>(B.prm);
+        const AllScalars<R> A = load_scalars<M, R>(B.prm, B.st);
         x_run<M, R, GRAV, COSMO, EXP>(B, A, E, c0, c1, T);
     }
     flush_counters_block(B.st, T, E.do_speed != 0);
@@ -580,6 +593,7 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
     double iv2[NV];                          // interface value at k-2 (left of cell k-2)
     unsigned bits = 0;                       // pair bits of k-3, k-2, k-1
     Face<double> fRprev;                     // right face of cell k-3
+    int cRprev = 0;                          // its clamp count
     Flux Fprev;                              // flux at interface k-3 (left of cell k-3)
 #pragma unroll
     for (int s = 0; s < NV; ++s) { w0[s] = w1[s] = w2[s] = w3[s] = iv2[s] = 0.0; d1[s] = d2[s] = R(0); }
@@ -624,17 +638,23 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
             for (int s = 0; s < NV; ++s) iv1[s] = M::iface(w2[s], w3[s], d2[s], d3[s]);
 
             Face<double> fL = fRprev, fR = fRprev;
+            int cL = 0, cR = 0;
             if (kk >= s0 + 1) {     // parabola and faces of cell k-2
                 const bool flat = flat_always || bits != 0u || bk;
                 M::template cell_faces<R, NV, GRAV, COSMO>(A, w2, iv2, iv1, w0[VR], w1[VR], w3[VR], cur[VR], d1[VR],
-                                                           d3[VR], w1[VP], w3[VP], flat, fL, fR, T.nclamp);
+                                                           d3[VR], w1[VP], w3[VP], flat, fL, fR, cL, cR);
             }
             Flux F = Fprev;
             if (kk >= s0 + 2) {     // Riemann problem at interface k-2
                 int nit;
                 F = M::riemann(A.Rs, fRprev, fL, nit);
                 peak_floor(A.Rs, F, r5, w0[VR], w1[VR], w2[VR], w0[VU], w1[VU]);
-                count_newton(nit, T.nitmax, T.nonconv);
+                // counted once per interface of the pencil (0 .. Nax): a
+                // segment owns interfaces s0 .. s1-1, the last one also Nax
+                if (kk - 2 < s1 || s1 == Nax) {
+                    count_newton(nit, T.nitmax, T.nonconv);
+                    T.nclamp += cRprev + cL;
+                }
             }
             if (kk >= s0 + 3) {     // update cell k-3
                 double pres = w1[VP], rho = w1[VR], u = w1[VU], vv = w1[VV], w = w1[VW];
@@ -657,6 +677,7 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
             }
             bits = (bits >> 1) | ((bk ? 1u : 0u) << 2);
             fRprev = fR;
+            cRprev = cR;
             Fprev = F;
         }
     }
@@ -675,7 +696,7 @@ __global__ void __launch_bounds__(256) sweep_stream(BoxView B, Epilogue E, int s
     const unsigned tt = active ? tid : 0u;
     const int seg = (int)(tt / (unsigned)npen);
     const long pen = (long)(tt - (unsigned)seg * (unsigned)npen);
-    const AllScalars<R> A = load_scalars<M, R>(B.prm);
+    const AllScalars<R> A = load_scalars<M, R>(B.prm, B.st);
     Tally T = tally0();
     stream_run<M, R, DIR, GRAV, COSMO, EXP>(B, A, E, pen, seg, seglen, active, T);
     flush_counters_block(B.st, T, E.do_speed != 0);

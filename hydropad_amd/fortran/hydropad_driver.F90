@@ -6,8 +6,13 @@
 ! (hydropad/main.F90:125-136 + io/write_var_mpi.F90: rho.bin, pressure.bin,
 ! vx.bin, vy.bin, vz.bin; REAL(4) interior, Fortran order, no header).
 !
-!   hydropad_driver N NSTEPS [fp64|mixed|exact] [OUTDIR] [resident|sweeps]
+!   hydropad_driver N NSTEPS [fp64|mixed|exact] [OUTDIR] [resident|sweeps] [NEWTON_LIMIT]
 !
+! NEWTON_LIMIT (default 0 = the reference's 100) is the Riemann Newton
+! iteration count past which the run stops as hydro/riemann.F90:60-65 does.
+! Mode "cells" runs no time loop: it calls the integrate_gpu drop-in on one
+! cold cell with equal face fluxes for the internal energies given as the
+! remaining arguments and prints each new pressure.
 ! "resident" keeps the state on the GPU for the whole run (attach /
 ! ppm_lxlylz_gpu / speed_gpu / detach); "sweeps" calls the per-sweep drop-in
 ! ppmsolver_gpu with host arrays exactly where hydro/ppm_lxlylz.F90 calls
@@ -40,6 +45,10 @@ program hydropad_driver
   end if
   if (command_argument_count() >= 4) call get_command_argument(4, outdir)
   if (command_argument_count() >= 5) call get_command_argument(5, mode)
+  if (command_argument_count() >= 6) then
+    call get_command_argument(6, arg); read(arg, *) i
+    call hydropad_gpu_set_newton_limit(i)
+  end if
 
   nb = 3
   nx = n + 2*nb; ny = nx; nz = nx
@@ -70,6 +79,11 @@ program hydropad_driver
 
   call hydropad_gpu_set_scalars(mingradflat, flatvalue, gamma, 1.0d0, m, rgamma1, rm, rgamma, dmax, &
                                 rdx, dt, dt, dx, 0.0d0, 1.0d0, gamma1, gf, eta1, eta2)
+  if (trim(mode) == 'cells') then
+    call integrate_cells()
+    call hydropad_gpu_finalize()
+    stop
+  end if
   if (trim(mode) == 'sweeps') then
     allocate(gforce(3, nx, ny, nz))
     gforce = 0.0d0
@@ -107,6 +121,22 @@ program hydropad_driver
   call hydropad_gpu_finalize()
 
 contains
+
+  ! integrate_gpu (SUBROUTINE integrate, hydro/integrate.F90:1-2) on a cold
+  ! cell, e/E ~ 1e-4 < eta1, with the caller's internal energy eint taken from
+  ! the command line (arguments 7, 8, ...).
+  subroutine integrate_cells()
+    integer :: a
+    real(c_double) :: eint, etot, pres, rho, vx, vy, vz
+    do a = 7, command_argument_count()
+      call get_command_argument(a, arg); read(arg, *) eint
+      pres = 1.0d-4; rho = 1.0d0; vx = 2.0d0; vy = 0.0d0; vz = 0.0d0
+      etot = pres * 1.5d0 + 0.5d0 * rho * vx * vx
+      call integrate_gpu(0.3d0, 0.3d0, 1.1d0, 1.1d0, 2.0d0, 2.0d0, 0.1d0, 0.1d0, -0.2d0, -0.2d0, &
+                         pres, rho, vx, vy, vz, etot, eint, 0.0d0)
+      write(*, '(a,es26.17e3,a,es26.17e3)') 'integrate: eint=', eint, ' pres=', pres
+    end do
+  end subroutine integrate_cells
 
   ! CFL maximum on the host, hydropad/speed.F90:22-31
   real(c_double) function host_speed()

@@ -33,13 +33,15 @@ module hydropad_gpu
 
   type, bind(C), public :: hp_counters_t
     integer(c_int64_t) :: newton_iters_max, newton_nonconv, neg_state_clamps
-    integer(c_int64_t) :: reserved(5)
+    integer(c_int64_t) :: particles_outside
+    integer(c_int64_t) :: reserved(4)
   end type hp_counters_t
 
   public :: hp_params_init, hp_create, hp_destroy, hp_set_precision, hp_upload, hp_download, &
             hp_set_params, hp_ppmsolver, hp_exchange_mesh, hp_ppm_lxlylz, hp_speed, hp_evolve, &
-            hp_status, hp_integrate_cell, hp_dualenergy_cell
-  public :: hydropad_gpu_set_scalars, hydropad_gpu_set_precision, ppmsolver_gpu, integrate_gpu, &
+            hp_status, hp_set_newton_limit, hp_integrate_cell, hp_dualenergy_cell
+  public :: hydropad_gpu_set_scalars, hydropad_gpu_set_precision, hydropad_gpu_set_newton_limit, &
+            ppmsolver_gpu, integrate_gpu, &
             dualenergy_gpu, hydropad_gpu_attach, ppm_lxlylz_gpu, speed_gpu, exchange_mesh_gpu, &
             hydropad_gpu_detach, hydropad_gpu_finalize, hydropad_gpu_check
 
@@ -107,6 +109,11 @@ module hydropad_gpu
       real(c_double), value :: cour, tstop
       real(c_double), intent(inout) :: t, dt
     end function
+    integer(c_int) function hp_set_newton_limit(ctx, nmax) bind(C, name="hp_set_newton_limit")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nmax
+    end function
     integer(c_int) function hp_status(ctx, c) bind(C, name="hp_status")
       import :: c_int, c_ptr, hp_counters_t
       type(c_ptr), value :: ctx
@@ -135,13 +142,20 @@ module hydropad_gpu
   integer, save :: cnx = 0, cny = 0, cnz = 0, cnb = 0
   integer(c_int), save :: cprec = HP_FP64
   type(hp_params_t), save :: prm
+  integer(c_int), save :: cnewt = 0
 
 contains
 
-  ! Stop with a message on a nonzero C-ABI return code.
+  ! Stop with a message on a nonzero C-ABI return code.  HP_E_NEWTON (-4) is
+  ! the reference's own stop on a Riemann Newton iteration that does not
+  ! converge, with its message (hydro/riemann.F90:60-65).
   subroutine hydropad_gpu_check(rc, what)
     integer(c_int), intent(in) :: rc
     character(len=*), intent(in) :: what
+    if (rc == -4_c_int) then
+      write(*,*) 'convergenza non raggiunta'
+      stop
+    end if
     if (rc /= 0) then
       write(*,*) 'hydropad_gpu: ', what, ' failed, code ', rc
       stop 1
@@ -166,6 +180,14 @@ contains
     if (c_associated(ctx)) call hydropad_gpu_check(hp_set_precision(ctx, cprec), 'hp_set_precision')
   end subroutine hydropad_gpu_set_precision
 
+  ! Newton iterations after which the run stops (the reference's nnewtmax = 100,
+  ! hydro/riemann.F90:51; 0 restores it).
+  subroutine hydropad_gpu_set_newton_limit(nmax)
+    integer, intent(in) :: nmax
+    cnewt = int(nmax, c_int)
+    if (c_associated(ctx)) call hydropad_gpu_check(hp_set_newton_limit(ctx, cnewt), 'hp_set_newton_limit')
+  end subroutine hydropad_gpu_set_newton_limit
+
   subroutine ensure_ctx(nx, ny, nz, nbound)
     integer, intent(in) :: nx, ny, nz, nbound
     if (c_associated(ctx) .and. (nx /= cnx .or. ny /= cny .or. nz /= cnz .or. nbound /= cnb)) then
@@ -176,6 +198,7 @@ contains
       call hydropad_gpu_check(hp_create(int(nx, c_int), int(ny, c_int), int(nz, c_int), &
                                         int(nbound, c_int), cprec, ctx), 'hp_create')
       cnx = nx; cny = ny; cnz = nz; cnb = nbound
+      call hydropad_gpu_check(hp_set_newton_limit(ctx, cnewt), 'hp_set_newton_limit')
     end if
     call hydropad_gpu_check(hp_set_params(ctx, prm), 'hp_set_params')
   end subroutine ensure_ctx

@@ -150,6 +150,35 @@ class Box:
         self.params.rdtath = dd.value
         return tt.value, dd.value
 
+    # ---- particles (nbody/) ----
+    def particles_upload(self, ppos, pvel, dtat):
+        """The reference's ppos/pvel as (np, 3) C-ordered float64 arrays (the
+        memory of its (3, np) arrays) and the kicks' time step dtat."""
+        arrs = []
+        for a in (ppos, pvel):
+            if a.ndim != 2 or a.shape[1] != 3 or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError("particle arrays must be float64 (np, 3) C-contiguous")
+            arrs.append(a)
+        if ppos.shape != pvel.shape:
+            raise ValueError("ppos and pvel differ in shape")
+        check("hp_particles_upload", self.L.hp_particles_upload(self._h, ppos.shape[0], _ptr(arrs[0]),
+                                                                _ptr(arrs[1]), float(dtat)))
+        self.npart = ppos.shape[0]
+
+    def particles_download(self):
+        n = getattr(self, "npart", 0)
+        x, v = np.empty((n, 3)), np.empty((n, 3))
+        check("hp_particles_download", self.L.hp_particles_download(self._h, _ptr(x), _ptr(v)))
+        return x, v
+
+    def kick(self, dtat):
+        """SUBROUTINE kick (nbody/kick.F90) on the resident particles."""
+        check("hp_kick", self.L.hp_kick(self._h, float(dtat)))
+
+    def drift(self):
+        """SUBROUTINE drift (nbody/drift.F90) with the current rdtath."""
+        check("hp_drift", self.L.hp_drift(self._h))
+
     # ---- comoving boxes ----
     def set_cosmo(self, sched):
         """Take the schedule's at, dat, rat, dt, rdtath as the sweep scalars."""
@@ -192,6 +221,11 @@ class Box:
 
     def reset_counters(self):
         check("hp_reset_counters", self.L.hp_reset_counters(self._h))
+
+    def set_newton_limit(self, nmax=0):
+        """Iterations after which a Riemann Newton iteration counts as not
+        converged (hydro/riemann.F90:51, nnewtmax = 100; 0 restores 100)."""
+        check("hp_set_newton_limit", self.L.hp_set_newton_limit(self._h, int(nmax)))
 
     def sync(self):
         check("hp_sync", self.L.hp_sync(self._h))

@@ -88,6 +88,20 @@ def uniform(N, rho=1.0, p=1.0, v=(0.0, 0.0, 0.0)):
     return st
 
 
+def smooth_wave(N, ny=3, nz=3, u0=0.5, amp=0.1):
+    """One period of a smooth subsonic wave along x in an N x ny x nz box:
+    rho = 1 + amp sin(kx), vx = u0 + amp sin(kx + 0.7), p = 1 + amp cos(kx),
+    k = 2 pi.  The mean flow keeps |v| > 0, so the kinetic share of the total
+    energy is never 0 and an eta1 of 1 sends every cell to the
+    internal-energy equation (hydro/integrate.F90:92-98)."""
+    st = empty_state((N + 2 * NB, ny + 2 * NB, nz + 2 * NB))
+    k = 2.0 * np.pi * cell_centres(N)
+    prof = {"rho": 1.0 + amp * np.sin(k), "vx": u0 + amp * np.sin(k + 0.7), "p": 1.0 + amp * np.cos(k)}
+    for name, v in prof.items():
+        st[name][NB:-NB, NB:-NB, NB:-NB] = v[:, None, None]
+    return periodic_fill(st)
+
+
 def random_smooth(N, seed, amp=0.3, vamp=0.3):
     """Seeded random state (rho, p in [1, 1+amp), v in [-vamp/2, vamp/2)):
     cell-to-cell noise that exercises every limiter branch."""
@@ -235,3 +249,52 @@ def grf_box(N, seed, slope=-2.0, rms=0.05, mach=50.0, vfac=None, a_start=1.0 / 5
     st["p"][sl] = rho * cs * cs / gamma
     st["vx"][sl], st["vy"][sl], st["vz"][sl] = vel
     return periodic_fill(st)
+
+
+def boxsize(ngrid):
+    """hydropad/init_sys.F90:93-95: each extent over the largest, in FP32."""
+    nmax = max(ngrid)
+    return tuple(float(np.float32(n) / np.float32(nmax)) for n in ngrid)
+
+
+def particles(ngrid, npart, seed, vamp=0.3):
+    """Seeded particles for the N-body step: positions uniform in the box
+    [0, boxsize) (the reference's ppos, hydropad/init_sys.F90:93-95 units),
+    velocities normal with deviation vamp.  Returns (ppos, pvel), each
+    (npart, 3) C-ordered float64 -- the memory layout of the reference's
+    (3, np) arrays (nbody/kick.F90:16-17)."""
+    r = np.random.default_rng(seed)
+    bs = np.array(boxsize(ngrid))
+    ppos = np.ascontiguousarray(r.random((npart, 3)) * bs)
+    ppos = np.minimum(ppos, np.nextafter(bs, 0.0))
+    pvel = np.ascontiguousarray(vamp * r.standard_normal((npart, 3)))
+    return ppos, pvel
+
+
+def gaussian_gforce(shape, nb=NB):
+    """The reference's static gravity set-up, hydropad/initialize_gravity.F90:25-58:
+    phi = exp(-|x - c|^2 / (2 sigma^2)) with c = (0.5, 0.5, 0.5), sigma = 0.3
+    (a default-REAL literal, so 0.3 rounded to FP32), cell centres
+    (real(i) - 0.5) dx taken in FP32 as there; gforce(d) = phi(i+1) - phi(i-1)
+    along d (no 1/(2 dx)).  Ghosts of phi and of gforce are periodic images
+    (the reference fills them with its halo exchange).  Returns (3, nx, ny, nz)
+    Fortran-ordered."""
+    nx, ny, nz = shape
+    ng = (nx - 2 * nb, ny - 2 * nb, nz - 2 * nb)
+    dx = float(np.float32(1.0) / np.float32(max(ng)))
+    sigma = float(np.float32(0.3))
+    cen = [(np.arange(1, n + 1, dtype=np.float32) - np.float32(0.5)).astype(np.float64) * dx - 0.5 for n in ng]
+    r2 = cen[0][:, None, None] ** 2 + cen[1][None, :, None] ** 2 + cen[2][None, None, :] ** 2
+    phi = np.zeros(shape, order="F")
+    phi[nb:-nb, nb:-nb, nb:-nb] = 1.0 * np.exp(-r2 / (2.0 * sigma ** 2))
+    periodic_fill({"phi": phi}, nb)
+    g = np.zeros((3,) + tuple(shape), order="F")
+    I = slice(nb, -nb)
+    g[0][I, I, I] = phi[nb + 1:nx - nb + 1, I, I] - phi[nb - 1:nx - nb - 1, I, I]
+    g[1][I, I, I] = phi[I, nb + 1:ny - nb + 1, I] - phi[I, nb - 1:ny - nb - 1, I]
+    g[2][I, I, I] = phi[I, I, nb + 1:nz - nb + 1] - phi[I, I, nb - 1:nz - nb - 1]
+    for c in range(3):
+        a = np.asfortranarray(g[c])
+        periodic_fill({"g": a}, nb)
+        g[c] = a
+    return g

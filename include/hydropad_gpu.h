@@ -14,8 +14,11 @@
  *    reference allocates p3d/rho3d/vx3d/vy3d/vz3d (hydropad/alloc_arrays.F90:30-32,56-60).
  *  - gforce, when given, is REAL(8) (3,nx,ny,nz) interleaved (hydropad/alloc_arrays.F90:89).
  *  - Every call returns 0 on success and a negative HP_E* code on failure; the
- *    reference's `stop` paths (Newton non-convergence, hydro/riemann.F90:62-65)
- *    become counters plus HP_E_NEWTON instead of terminating the process.
+ *    reference's `stop` on Newton non-convergence (hydro/riemann.F90:60-65)
+ *    becomes a device counter, and hp_ppmsolver, hp_ppm_lxlylz, hp_evolve and
+ *    hp_evolve_cosmo return HP_E_NEWTON when it is non-zero (it counts since
+ *    the last hp_reset_counters; the state past such a step is what the
+ *    reference would never have produced).
  *  - A context owns its device state and one HIP stream; calls on one context
  *    are stream-ordered and must come from one host thread at a time.
  */
@@ -34,7 +37,7 @@ extern "C" {
 #define HP_E_ARG        -1   /* bad argument (null pointer, bad size, bad direction) */
 #define HP_E_HIP        -2   /* a HIP runtime call failed */
 #define HP_E_NOMEM      -3   /* device allocation failed / budget exceeded */
-#define HP_E_NEWTON     -4   /* Riemann Newton iteration exceeded 100 steps (hydro/riemann.F90:62-65) */
+#define HP_E_NEWTON     -4   /* Riemann Newton iteration exceeded 100 steps (hydro/riemann.F90:60-65) */
 #define HP_E_STATE      -5   /* call not valid in the context's current state */
 
 /* ---- precision forms ---------------------------------------------------- */
@@ -61,7 +64,8 @@ typedef struct hp_counters {
     int64_t newton_iters_max;   /* largest Newton iteration count seen */
     int64_t newton_nonconv;     /* interfaces whose Newton loop hit 100 iterations */
     int64_t neg_state_clamps;   /* traced p/rho < 0 clamped to 1e-15 */
-    int64_t reserved[5];
+    int64_t particles_outside;  /* kicks skipped: the particle's CIC cube leaves the array */
+    int64_t reserved[4];
 } hp_counters;
 
 typedef struct hp_ctx hp_ctx;
@@ -184,9 +188,41 @@ int hp_set_tiling(hp_ctx* ctx, int x_chunks, int yz_seglen, int block);
 int hp_sweep_timing_enable(hp_ctx* ctx, int on);
 int hp_sweep_timing(hp_ctx* ctx, double ms[3], int64_t launches[3]);
 
-/* Accumulated counters since the last hp_reset_counters. */
+/* ---- particles (the N-body step beside the hydro step) -----------------------
+ * The reference's ppos/pvel, (3,np) REAL(8) interleaved (nbody/kick.F90:16-17),
+ * in the units of hydropad/init_sys.F90:93-95 (the box is [0,boxsize) with
+ * boxsize = ngrid/max(ngrid) in FP32).  Particles with ppos(1) == -1 are
+ * inactive (kick.F90:37).  dtat is the kicks' time step, the reference's
+ * module scalar dtat (hydropad/scalar.F90:18), which it sets once at
+ * initialisation (initialize_parameters.F90:174,193) and never updates; the
+ * time loops below keep it fixed in the same way.  When particles are present,
+ * hp_evolve and hp_evolve_cosmo run kick, drift, kick after the sweeps of each
+ * step (hydropad/evolve_sys.F90:63-84, one rank) with the context's gforce
+ * (zero without one), and hp_speed and the loops' CFL maximum include max |v|
+ * of every particle (hydropad/speed.F90:43-56). */
+int hp_particles_upload(hp_ctx* ctx, int64_t np, const double* ppos, const double* pvel, double dtat);
+int hp_particles_download(hp_ctx* ctx, double* ppos, double* pvel);
+
+/* SUBROUTINE kick (nbody/kick.F90:1-133): v <- v (1 - dat dtat/2) + F dtat/2,
+ * F the cell-in-cell interpolation of the context's gforce (the sweeps'
+ * scalars supply dat).  A particle whose interpolation cube would leave the
+ * array is not kicked and is counted in hp_counters.particles_outside. */
+int hp_kick(hp_ctx* ctx, double dtat);
+
+/* SUBROUTINE drift (nbody/drift.F90:1-36): x <- x + rdtath v, wrapped once
+ * into [0, boxsize). */
+int hp_drift(hp_ctx* ctx);
+
+/* Accumulated counters since the last hp_reset_counters; returns HP_E_NEWTON
+ * when newton_nonconv is non-zero. */
 int hp_status(hp_ctx* ctx, hp_counters* out);
 int hp_reset_counters(hp_ctx* ctx);
+
+/* Iteration count past which a Riemann Newton iteration counts as not
+ * converged: the reference's local nnewtmax = 100 (hydro/riemann.F90:51,60-65).
+ * nmax = 0 restores 100.  A smaller limit makes the non-convergence path
+ * reachable in tests (finite positive states converge well within 100). */
+int hp_set_newton_limit(hp_ctx* ctx, int nmax);
 int hp_sync(hp_ctx* ctx);
 
 /* ---- batched ensemble ----------------------------------------------------------
@@ -242,6 +278,9 @@ int hp_batch_add_sod(hp_batch* b, int n, int precision, uint64_t seed, double co
 int hp_sod_of_seed(uint64_t seed, int* axis, double* x0, int* flip);
 
 int hp_batch_remove(hp_batch* b, int id);
+
+/* hp_set_newton_limit for one box of the batch (0 restores the reference's 100). */
+int hp_batch_set_newton_limit(hp_batch* b, int id, int nmax);
 int hp_batch_download(hp_batch* b, int id, double* p, double* rho, double* vx, double* vy, double* vz);
 
 /* Enqueue nsteps batched steps (asynchronous); boxes that reached their step
@@ -277,7 +316,8 @@ int hp_batch_autotune(hp_batch* b, int steps_per_trial, hp_batch_tuning* best,
 typedef struct hp_request {
     int64_t id;
     int32_t n, nsteps;
-    int32_t precision, pad;
+    int32_t precision;
+    int32_t newton_limit;   /* 0: the reference's 100 (see hp_set_newton_limit) */
     uint64_t seed;
     double arrival_s;
     double cour;
@@ -305,6 +345,8 @@ int hp_serve_run(const hp_request* reqs, int nreq, size_t hbm_budget, int max_bo
 /* ---- scalar entry points (host-callable, for unit vectors) ---------------- */
 /* SUBROUTINE integrate (hydro/integrate.F90:1-113) on one cell, arguments by
  * pointer in the reference's order; pres, rho, vx, vy, vz are updated in place.
+ * The caller's etot and eint are used as given (the internal-energy equation
+ * and its fallbacks, integrate.F90:75-81,102-105, start from eint).
  * pmean, undefined in the reference's active build, is (pm1+pm2)*0.5 as in
  * hydro/ppm.F90:583. */
 int hp_integrate_cell(const hp_params* prm,

@@ -41,7 +41,8 @@ class HpParams(ctypes.Structure):
 
 class HpCounters(ctypes.Structure):
     _fields_ = [("newton_iters_max", ctypes.c_int64), ("newton_nonconv", ctypes.c_int64),
-                ("neg_state_clamps", ctypes.c_int64), ("reserved", ctypes.c_int64 * 5)]
+                ("neg_state_clamps", ctypes.c_int64), ("particles_outside", ctypes.c_int64),
+                ("reserved", ctypes.c_int64 * 4)]
 
 
 COSMO_NAMES = ("cour", "ca", "tstop", "t", "dt", "at", "dat", "ath", "dath", "atnew", "datnew", "rat", "rdtath")
@@ -112,12 +113,45 @@ class Oracle:
         L.or_expand.restype = None
         L.or_evolve_cosmo.argtypes = [PP] + [_P] * 6 + [_I] * 6 + [PC, ctypes.POINTER(HpCounters)]
         L.or_evolve_cosmo.restype = _I
+        L.or_kick.argtypes = [PP, ctypes.c_int64, _P, _P, _P, _I, _I, _I, _I, _D]
+        L.or_kick.restype = ctypes.c_int64
+        L.or_drift.argtypes = [PP, ctypes.c_int64, _P, _P, _I, _I, _I]
+        L.or_drift.restype = None
+        L.or_particle_vmax.argtypes = [ctypes.c_int64, _P]
+        L.or_particle_vmax.restype = _D
         L.or_set_threads.argtypes = [_I]
+        L.or_set_newton_max.argtypes = [_I]
+        L.or_plant.argtypes = [_I]
         L.or_nrot_dir.argtypes = [_I, _I]
         L.or_nrot_dir.restype = _I
 
+    # ---- N-body: ppos, pvel are (3, np) Fortran-ordered (np, 3 C-contiguous) ----
+    def kick(self, P, ppos, pvel, gforce, dtat, nbound=3):
+        """nbody/kick.F90 in place on pvel; returns the particles skipped
+        because their CIC cube leaves the array."""
+        g = np.asfortranarray(gforce, dtype=np.float64)
+        _, nx, ny, nz = g.shape
+        return self.L.or_kick(ctypes.byref(P), ppos.shape[0], _ptr(ppos), _ptr(pvel), _ptr(g), nx, ny, nz,
+                              nbound, dtat)
+
+    def drift(self, P, ppos, pvel, ngrid):
+        """nbody/drift.F90 in place on ppos; ngrid = (ngridx, ngridy, ngridz)."""
+        self.L.or_drift(ctypes.byref(P), ppos.shape[0], _ptr(ppos), _ptr(pvel), *ngrid)
+
+    def particle_vmax(self, pvel):
+        return self.L.or_particle_vmax(pvel.shape[0], _ptr(pvel))
+
     def set_threads(self, n):
         self.L.or_set_threads(int(n))
+
+    def plant(self, what=0):
+        """Plant a known mistake (tests that show a check can fail): 1 pmean =
+        pm1, 2 pmean = pm1 + pm2, 3 pmean = -(pm1 + pm2)/2; 0 removes it."""
+        self.L.or_plant(int(what))
+
+    def set_newton_max(self, n=0):
+        """Newton iteration limit (hydro/riemann.F90:51 has 100; 0 restores it)."""
+        self.L.or_set_newton_max(int(n))
 
     def params(self, gamma=5.0 / 3.0, ngrid=64, dt=0.0):
         p = HpParams()
@@ -245,6 +279,9 @@ class Reference:
         L.ref_riemannsolver.argtypes = [_P] * 4
         L.ref_riemann.argtypes = [_P] + [_D] * 6
         L.ref_integrate.argtypes = [_P] * 3
+        L.ref_nbody_setup.argtypes = [_I] * 4
+        L.ref_kick.argtypes = [_I, _P, _P, _I, _I, _I, _P, _D, _D]
+        L.ref_drift.argtypes = [_I, _P, _P, _D]
 
     @staticmethod
     def available():
@@ -277,6 +314,16 @@ class Reference:
         e = np.array([etot, eint, ghalf])
         self.L.ref_integrate(_ptr(f), _ptr(st), _ptr(e))
         return st
+
+    def kick(self, ppos, pvel, gforce, dat, dtat, nbound=3):
+        g = np.asfortranarray(gforce, dtype=np.float64)
+        _, nx, ny, nz = g.shape
+        self.L.ref_nbody_setup(nx - 2 * nbound, ny - 2 * nbound, nz - 2 * nbound, nbound)
+        self.L.ref_kick(ppos.shape[0], _ptr(ppos), _ptr(pvel), nx, ny, nz, _ptr(g), dat, dtat)
+
+    def drift(self, ppos, pvel, rdtath, ngrid, nbound=3):
+        self.L.ref_nbody_setup(*ngrid, nbound)
+        self.L.ref_drift(ppos.shape[0], _ptr(ppos), _ptr(pvel), rdtath)
 
     def ppmsolver(self, state, ndir, nbound=3, gforce=None):
         f = Oracle._fields(state)

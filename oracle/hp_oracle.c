@@ -19,9 +19,13 @@
  *     by oracle/Makefile into oracle/_ref (and frozen under tests/golden/);
  *   - against closed forms: the exact Sod star state, Rankine-Hugoniot,
  *     linear/constant profiles, uniform-state invariance, conservation.
- * Parity unpinned by the reference: integrate's `pmean` (undefined in the
- * reference's active build, hydro/integrate.F90:19,186); it is taken as
- * (pm1+pm2)*0.5 from hydro/ppm.F90:583 and pinned only by closed forms.
+ * Not pinned by the reference: integrate's `pmean`, an uninitialised local in
+ * the reference's active build (hydro/integrate.F90:19,60-75; its flang build
+ * returns a NaN pressure in that regime).  It is taken as (pm1+pm2)*0.5 from
+ * hydro/ppm.F90:583 and pinned by convergence (tests/test_oracle_dual_energy.py):
+ * with every cell on the internal-energy equation (e/E < eta1) a smooth wave
+ * must approach the total-energy run at second order, which a one-sided,
+ * doubled or sign-flipped pmean (planted with or_plant) fails.
  */
 #include <math.h>
 #include <stdint.h>
@@ -312,14 +316,19 @@ static double rhoshock(const hp_params* P, double p2, double p1, double rho1)
     return rho1 * (P->gamma1 * p2 / p1 + 1.0) / (P->gamma1 + p2 / p1);
 }
 
+/* The reference's nnewtmax = 100 (hydro/riemann.F90:51); settable so that
+ * tests can reach the non-convergence path with ordinary states. */
+static int newton_max = 100;
+void or_set_newton_max(int n) { newton_max = n > 0 ? n : 100; }
+
 /* hydro/riemann.F90:29-111 (van Leer Newton iteration).  Returns the number
- * of iterations, or -1 where the reference would `stop` (>100). */
+ * of iterations, or -1 where the reference would `stop` (>nnewtmax). */
 int or_riemann(const hp_params* P, double* pstar, double* vstar, double pleft, double pright,
                double rholeft, double rhoright, double vxleft, double vxright)
 {
     const double gamma = P->gamma, m = P->m, gf = P->gf;
     double err = 1.e10, terr = F_TERR, a = 0, b = 0;
-    int nnewt = 0, nnewtmax = 100;
+    int nnewt = 0, nnewtmax = newton_max;
     double kappa = sqrt(gamma * (pleft + pright) * (rholeft + rhoright) * 0.25);
     double pcheck = 0.50 * (pleft + pright) + 0.50 * kappa * (vxleft + vxright);
     if (pcheck > 0) *pstar = pcheck;
@@ -486,6 +495,11 @@ double or_total(const hp_params* P, double pres, double rho, double vx, double v
     return pres * P->rgamma1 + 0.5 * rho * (vx * vx + vy * vy + vz * vz);
 }
 
+/* Planted mistakes for the tests that show a check can fail (0 = none):
+ * 1 pmean = pm1, 2 pmean = pm1 + pm2, 3 pmean = -(pm1 + pm2)/2. */
+static int planted = 0;
+void or_plant(int what) { planted = what; }
+
 /* hydro/integrate.F90:1-113; pmean from hydro/ppm.F90:583. */
 void or_integrate(const hp_params* P, double pm1, double pm2, double rhom1, double rhom2, double vxm1,
                   double vxm2, double vym1, double vym2, double vzm1, double vzm2, double* pres,
@@ -511,6 +525,9 @@ void or_integrate(const hp_params* P, double pm1, double pm2, double rhom1, doub
     grav = (*rho * *vx + rhonew * vxnew) * ghalf * 0.5;
     etotnew = etot + rdtath * (flux1 - flux2) * rdx + rdtath * grav;
     double pmean = (pm1 + pm2) * 0.5;
+    if (planted == 1) pmean = pm1;
+    else if (planted == 2) pmean = pm1 + pm2;
+    else if (planted == 3) pmean = -(pm1 + pm2) * 0.5;
     eintnew = eint + rdtath * pmean * (vxm1 - vxm2) * rdx;
     flux1 = (pm1 * vxm1) * rgamma1;
     flux2 = (pm2 * vxm2) * rgamma1;
@@ -837,4 +854,89 @@ int or_evolve_cosmo(hp_params* P, double* p, double* rho, double* vx, double* vy
     P->at = s->at; P->dat = s->dat; P->rat = s->rat; P->dt = s->dt; P->rdtath = s->rdtath;
     free(old);
     return worst;
+}
+
+/* ---- N-body (nbody/kick.F90, nbody/drift.F90, hydropad/speed.F90:43-56) ---- */
+
+/* Cell-in-cell interpolation of gforce (3,nx,ny,nz) at a particle position,
+ * nbody/kick.F90:47-117 on one rank (boxlaux = 0).  ng* are the interior
+ * sizes (the reference's ngridx..z).  Returns 0 when the CIC cube would leave
+ * the array (the reference would read outside it). */
+static int or_cic(const double* gforce, int nx, int ny, int nz, int nb, int ngx, int ngy, int ngz,
+                  const double* pos, double* ff)
+{
+    double x1g = (pos[0] - (double)0.0f) * (double)(float)ngx;
+    double x2g = (pos[1] - (double)0.0f) * (double)(float)ngy;
+    double x3g = (pos[2] - (double)0.0f) * (double)(float)ngz;
+    /* leading (left) cell, 1-based as the reference: int() truncates toward 0 */
+    const double t1 = x1g + 0.5, t2 = x2g + 0.5, t3 = x3g + 0.5;
+    if (!(t1 > -1.0 && t2 > -1.0 && t3 > -1.0 && t1 < 1e9 && t2 < 1e9 && t3 < 1e9)) return 0;
+    int i1 = (int)t1 + nb, j1 = (int)t2 + nb, k1 = (int)t3 + nb;
+    if (i1 < 1 || j1 < 1 || k1 < 1 || i1 + 1 > nx || j1 + 1 > ny || k1 + 1 > nz) return 0;
+    double xi1 = (double)((float)(i1 - nb) - 0.5f);
+    double xi2 = (double)((float)(j1 - nb) - 0.5f);
+    double xi3 = (double)((float)(k1 - nb) - 0.5f);
+    double dd1 = x1g - xi1, dd2 = x2g - xi2, dd3 = x3g - xi3;
+    double de1 = 1.0 - dd1, de2 = 1.0 - dd2, de3 = 1.0 - dd3;
+    double d1 = de1 * de2 * de3, d2 = dd1 * de2 * de3, d3 = de1 * dd2 * de3, d4 = dd1 * dd2 * de3;
+    double d5 = de1 * de2 * dd3, d6 = dd1 * de2 * dd3, d7 = de1 * dd2 * dd3, d8 = dd1 * dd2 * dd3;
+#define GF(c, i, j, k) gforce[(c) + 3 * ((size_t)((i) - 1) + (size_t)nx * ((size_t)((j) - 1) + (size_t)ny * ((k) - 1)))]
+    for (int c = 0; c < 3; ++c)
+        ff[c] = GF(c, i1, j1, k1) * d1 + GF(c, i1 + 1, j1, k1) * d2 + GF(c, i1, j1 + 1, k1) * d3 +
+                GF(c, i1 + 1, j1 + 1, k1) * d4 + GF(c, i1, j1, k1 + 1) * d5 + GF(c, i1 + 1, j1, k1 + 1) * d6 +
+                GF(c, i1, j1 + 1, k1 + 1) * d7 + GF(c, i1 + 1, j1 + 1, k1 + 1) * d8;
+#undef GF
+    return 1;
+}
+
+/* SUBROUTINE kick, nbody/kick.F90:1-133: half-step velocity update
+ * v <- v (1 - dat dtat/2) + F dtat/2 with F interpolated by CIC.  Particles
+ * with x == -1 are inactive (:37).  Returns the number of particles skipped
+ * because their CIC cube leaves the array. */
+int64_t or_kick(const hp_params* P, int64_t np, const double* ppos, double* pvel, const double* gforce, int nx,
+                int ny, int nz, int nb, double dtat)
+{
+    const double dat = P->dat;
+    const int ngx = nx - 2 * nb, ngy = ny - 2 * nb, ngz = nz - 2 * nb;
+    int64_t outside = 0;
+    for (int64_t j = 0; j < np; ++j) {
+        const double* x = ppos + 3 * j;
+        double* v = pvel + 3 * j;
+        if (x[0] == -1) continue;
+        double ff[3];
+        if (!or_cic(gforce, nx, ny, nz, nb, ngx, ngy, ngz, x, ff)) { ++outside; continue; }
+        for (int c = 0; c < 3; ++c) v[c] = v[c] * (1.0 - 0.5 * dat * dtat) + 0.5 * ff[c] * dtat;
+    }
+    return outside;
+}
+
+/* boxsize of hydropad/init_sys.F90:93-95: ratios taken in FP32 */
+static double or_boxsize(int n, int nmax) { return (double)((float)n / (float)nmax); }
+
+/* SUBROUTINE drift, nbody/drift.F90:1-36: x <- x + rdtath v, wrapped once
+ * into [0, boxsize). */
+void or_drift(const hp_params* P, int64_t np, double* ppos, const double* pvel, int ngx, int ngy, int ngz)
+{
+    const int nmax = ngx > ngy ? (ngx > ngz ? ngx : ngz) : (ngy > ngz ? ngy : ngz);
+    const double bs[3] = {or_boxsize(ngx, nmax), or_boxsize(ngy, nmax), or_boxsize(ngz, nmax)};
+    for (int64_t j = 0; j < np; ++j) {
+        double* x = ppos + 3 * j;
+        const double* v = pvel + 3 * j;
+        if (x[0] == -1) continue;
+        for (int c = 0; c < 3; ++c) x[c] = x[c] + P->rdtath * v[c];
+        for (int c = 0; c < 3; ++c) {
+            if (x[c] < 0.0) x[c] = x[c] + bs[c];
+            if (x[c] >= bs[c]) x[c] = x[c] - bs[c];
+        }
+    }
+}
+
+/* The particle term of SUBROUTINE speed, hydropad/speed.F90:43-56: the
+ * largest |v| component of every particle (inactive ones included). */
+double or_particle_vmax(int64_t np, const double* pvel)
+{
+    double vmax = 0.0;
+    for (int64_t i = 0; i < np; ++i)
+        vmax = dmax2(vmax, dmax2(dmax2(fabs(pvel[3 * i]), fabs(pvel[3 * i + 1])), fabs(pvel[3 * i + 2])));
+    return vmax;
 }

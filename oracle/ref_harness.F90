@@ -98,4 +98,59 @@ contains
          st(1), st(2), st(3), st(4), st(5), ee(1), ee(2), ee(3))
   end subroutine ref_integrate
 
+  ! The N-body set-up kick and drift read (hydropad/init_sys.F90:75-100 on one
+  ! rank): the mesh sizes, one rank per dimension, boxsize (FP32 ratios, as
+  ! there) and the periodic Cartesian communicator MPI_CART_RANK needs.
+  subroutine ref_nbody_setup(ngx, ngy, ngz, nb) bind(C, name="ref_nbody_setup")
+    use dimension
+    use mpi_inc
+    integer(c_int), value :: ngx, ngy, ngz, nb
+    integer :: d(3), e
+    logical :: per(3)
+    call ref_mpi_init()
+    ngridx = ngx; ngridy = ngy; ngridz = ngz
+    ngridxpe = ngx; ngridype = ngy; ngridzpe = ngz
+    npesx = 1; npesy = 1; npesz = 1
+    nbound = nb
+    if (.not. allocated(boxsize)) allocate(boxsize(3))
+    boxsize(1) = real(ngridx)/real(max(ngridx,ngridy,ngridz))
+    boxsize(2) = real(ngridy)/real(max(ngridx,ngridy,ngridz))
+    boxsize(3) = real(ngridz)/real(max(ngridx,ngridy,ngridz))
+    if (COMM_CART == 0) then
+      d = 1; per = .true.
+      call MPI_Cart_create(MPI_COMM_WORLD, 3, d, per, 0, COMM_CART, e)
+    end if
+  end subroutine ref_nbody_setup
+
+  ! SUBROUTINE kick (nbody/kick.F90:1-133) on rank 0 with the module scalars
+  ! dat and dtat; ppos/pvel are (3,np), gforce (3,nx,ny,nz).
+  subroutine ref_kick(np, ppos, pvel, nx, ny, nz, gforce, datv, dtatv) bind(C, name="ref_kick")
+    use scalar
+    integer(c_int), value :: np, nx, ny, nz
+    real(c_double), intent(in) :: ppos(3, np), gforce(3, nx, ny, nz)
+    real(c_double), intent(inout) :: pvel(3, np)
+    real(c_double), value :: datv, dtatv
+    integer :: n, ix, iy, iz, keep
+    external :: kick
+    n = np; ix = nx; iy = ny; iz = nz
+    dat = datv; dtat = dtatv
+    keep = mype
+    mype = 0
+    call kick(n, ppos, pvel, ix, iy, iz, gforce)
+    mype = keep
+  end subroutine ref_kick
+
+  ! SUBROUTINE drift (nbody/drift.F90:1-36) with the module scalar rdtath.
+  subroutine ref_drift(np, ppos, pvel, rdtathv) bind(C, name="ref_drift")
+    use scalar
+    integer(c_int), value :: np
+    real(c_double), intent(inout) :: ppos(3, np), pvel(3, np)
+    real(c_double), value :: rdtathv
+    integer :: n
+    external :: drift
+    n = np
+    rdtath = rdtathv
+    call drift(n, ppos, pvel)
+  end subroutine ref_drift
+
 end module hp_ref_harness

@@ -18,6 +18,10 @@ Files (all float64, numpy .npz):
                     periodic boxes (random, Sod, gravity, expansion terms)
   steps.npz         three full steps (ppmsolver x3 in nrot order, periodic
                     fill between sweeps, speed + timedet) of a 12^3 box
+  nbody.npz         nbody/kick.F90 and nbody/drift.F90 on seeded particles
+                    (inactive ones and periodic wraps included) over random
+                    gforce arrays of a cubic and a non-cubic box
+                    (written alone by `gen_golden.py nbody`)
 """
 import os
 import sys
@@ -262,17 +266,53 @@ def gen_steps(O, R, rng):
                         dts=np.array(dts), t=np.array([t]), cour=np.array([cour]))
 
 
+def nbody_case(rng, ngrid, npart):
+    """Random gforce (ghosts included), particles inside [0, boxsize) plus a
+    few at the box edges and inactive ones (x = -1)."""
+    shape = tuple(n + 2 * NB for n in ngrid)
+    g = np.asfortranarray(rng.standard_normal((3,) + shape))
+    bs = np.array([np.float32(n) / np.float32(max(ngrid)) for n in ngrid], dtype=np.float64)
+    x = rng.random((npart, 3)) * bs
+    x[0] = 0.0
+    x[1] = np.nextafter(bs, 0.0)
+    x[2] = 0.5 * bs
+    x[3:6, 0] = -1.0
+    v = rng.standard_normal((npart, 3))
+    v[6] = -2.0 * bs                       # drifts out below 0 and wraps
+    v[7] = 2.0 * bs                        # and above boxsize
+    return g, np.ascontiguousarray(x), np.ascontiguousarray(v)
+
+
+def gen_nbody(O, R, rng):
+    out = {}
+    for name, ngrid in (("cubic", (8, 8, 8)), ("noncubic", (6, 8, 5))):
+        g, x, v = nbody_case(rng, ngrid, 120)
+        dat, dtat, rdtath = 0.35, 0.02, 0.05
+        vk = v.copy()
+        R.kick(x, vk, g, dat, dtat)
+        xd = x.copy()
+        R.drift(xd, v * 0.0 + v, rdtath, ngrid)
+        out.update({name + "_gforce": g, name + "_ppos": x, name + "_pvel": v, name + "_kick": vk,
+                    name + "_drift": xd, name + "_scal": np.array([dat, dtat, rdtath]),
+                    name + "_ngrid": np.array(ngrid)})
+    np.savez_compressed(os.path.join(OUT, "nbody.npz"), **out)
+
+
 def main():
     O = Oracle()
     if not Reference.available():
         raise SystemExit("oracle/_ref not built (needs the reference tree): make -C oracle ref")
     R = Reference()
+    if sys.argv[1:] == ["nbody"]:
+        gen_nbody(O, R, np.random.default_rng(20261203))
+        return
     rng = np.random.default_rng(20261202)
     gen_order2interp(O, R, rng)
     gen_riemann(O, R, rng)
     gen_integrate(O, R, rng)
     gen_sweeps(O, R, rng)
     gen_steps(O, R, rng)
+    gen_nbody(O, R, np.random.default_rng(20261203))
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

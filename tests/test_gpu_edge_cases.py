@@ -1,0 +1,266 @@
+"""GPU parity on the reference's edge cases, through the C-ABI:
+
+* the Riemann solver's vacuum pattern (riemannsolver.F90:231-265) and its
+  high-density-peak pressure floor (:279-285), which no ordinary flow reaches;
+* the negative traced-state clamp of order2interp.F90:174-190;
+* the internal-energy branch of integrate (e/E < eta1) on a whole box, and
+  integrate with the caller's own eint (integrate.F90:1-2,75-81,102-105);
+* the reference's `stop` on Newton non-convergence (riemann.F90:60-65) as
+  HP_E_NEWTON from every stepping entry point, the serving loop and the
+  Fortran drop-in.  The iteration limit is lowered from the host
+  (hp_set_newton_limit): finite positive states converge well inside the
+  reference's 100 iterations, so no input reaches it otherwise.  Kernels exit
+  normally; only the return codes and counters are checked.
+
+Bars: FP64 forms relative L1 <= 1e-12 per field against the oracle, the mixed
+form L1(rho) <= 1e-5; clamp and non-convergence counts equal the oracle's.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hydropad_amd as hp
+from helpers import FIELDS, NB, copy_state, rel_l1, rel_linf, to_hp_params, x_interfaces
+from hydropad_amd import _capi, ic
+from hydropad_amd.batch import serve
+from oracle import HpCounters as OracleCounters
+
+pytestmark = pytest.mark.gpu
+
+TOL_L1 = 1e-12
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "hydropad_amd", "fortran", "hydropad_driver")
+
+
+def gpu_sweep(st, P, d, precision):
+    with hp.Box(st["p"].shape, precision) as box:
+        box.set_params(to_hp_params(P, hp.HpParams))
+        box.upload(copy_state(st))
+        box.ppmsolver(d)
+        out = box.download()
+        c = box.counters()
+    return out, c
+
+
+def check(out, ref, precision, what):
+    if precision == "mixed":
+        assert rel_l1(out["rho"], ref["rho"]) <= 1e-5, what
+        return
+    for k in FIELDS:
+        e = rel_l1(out[k], ref[k])
+        assert e <= TOL_L1, (what, k, e, rel_linf(out[k], ref[k]))
+
+
+def vacuum_state(N=16):
+    """Two streams pulling apart 5 % faster than two rarefactions can follow:
+    dvv = -6.3 c < dv2r = -6 c at the interfaces of the velocity jumps."""
+    st = ic.uniform(N, rho=1.0, p=1e-2)
+    v0 = 3.0 * np.sqrt(5.0 / 3.0 * 1e-2) * 1.05
+    x = ic.cell_centres(N)
+    st["vx"][NB:-NB] = np.where(x < 0.5, -v0, v0)[:, None, None]
+    return ic.periodic_fill(st)
+
+
+def peak_state(N=16):
+    """Two-cell density peaks (5x their neighbours) in a random velocity field,
+    swept with dmax = 2 so that the floor's density test passes."""
+    rng = np.random.default_rng(5)
+    st = ic.uniform(N, rho=1.0, p=1.0)
+    for i in (4, 5, 10, 11):
+        st["rho"][NB + i] = 5.0
+    st["vx"][NB:-NB] = (0.3 * rng.standard_normal(N))[:, None, None]
+    return ic.periodic_fill(st)
+
+
+def case_params(oracle, name, st, N=16):
+    P = oracle.params(ngrid=N)
+    if name == "vacuum":
+        P.dt, P.rdtath = 1e-3, 1e-3
+        return P
+    if name == "peak":
+        P.dmax = 2.0
+    vm = oracle.speed(P, st)
+    oracle.timedet(P, vm, 0.8, 1e30, 0.0)
+    return P
+
+
+EDGE = [("vacuum", vacuum_state), ("peak", peak_state)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64_exact", "mixed"])
+@pytest.mark.parametrize("name,make", EDGE, ids=[e[0] for e in EDGE])
+def test_riemann_edge_cases_vs_oracle(oracle, name, make, precision):
+    print(f"edge case {name} {precision}", flush=True)
+    st = make()
+    P = case_params(oracle, name, st)
+    ref = copy_state(st)
+    assert oracle.ppmsolver(P, ref, 1) >= 0
+    out, c = gpu_sweep(st, P, 1, precision)
+    assert c.newton_nonconv == 0
+    check(out, ref, precision, name)
+
+
+def test_edge_inputs_reach_every_wave_pattern_and_the_floor(oracle):
+    """The inputs of the GPU edge-case tests, classified with the reference's
+    own tests (dvv vs dv2s, dv1s, dv2r; the dmax floor raising p*)."""
+    seen, floors = set(), 0
+    for name, make in EDGE + [("sod", lambda: ic.sod(16))]:
+        st = make()
+        P = case_params(oracle, name, st)
+        for _, _, case, floor in x_interfaces(oracle, P, st):
+            seen.add(case)
+            floors += floor
+    assert seen == {1, 2, 3, 4}, seen
+    assert floors > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64_exact"])
+def test_negative_state_clamp_vs_oracle(oracle, precision):
+    """A strong random state swept at Courant number 4: the parabola is
+    averaged past the cell, the traced p or rho goes negative and is clamped
+    to 1e-15 (order2interp.F90:174-190) at the same interfaces as the oracle.
+    Only a Courant number above 1 reaches the clamp (below it the traced value
+    is an average over part of the cell's monotone parabola).  The
+    reference-order form matches the oracle there to the FP64 bar; the
+    re-associated FP64 form clamps the same interfaces, but the star states
+    behind a 1e-15 pressure are ill-conditioned and its fields differ
+    (about 3e-4 relative L1 in p), so only the count and finiteness are held
+    to the oracle for it."""
+    print(f"clamp {precision}", flush=True)
+    N = 16
+    st = ic.random_smooth(N, 3, amp=3.0, vamp=2.0)
+    P = oracle.params(ngrid=N)
+    oracle.timedet(P, oracle.speed(P, st), 4.0, 1e30, 0.0)
+    ref = copy_state(st)
+    cnt = OracleCounters()
+    oracle.ppmsolver(P, ref, 1, counters=cnt)
+    assert cnt.neg_state_clamps > 0
+    out, c = gpu_sweep(st, P, 1, precision)
+    assert c.neg_state_clamps == cnt.neg_state_clamps
+    if precision == "fp64_exact":
+        check(out, ref, precision, "clamp")
+    else:
+        for k in FIELDS:
+            assert np.isfinite(out[k]).all(), k
+
+
+def test_internal_energy_branch_on_a_box(oracle):
+    """The smooth wave with every cell on the internal-energy equation
+    (eta1 = 1, eta2 = 2, e/E < 1 everywhere), 20 fixed-dt steps."""
+    N, K = 32, 20
+    st = ic.smooth_wave(N)
+    P = oracle.params(ngrid=N)
+    P.eta1, P.eta2 = 1.0, 2.0
+    P.dt = P.rdtath = 0.4 * P.dx / 2.0
+    ref = copy_state(st)
+    for n in range(1, K + 1):
+        assert oracle.ppm_lxlylz(P, ref, 1 if n == 1 else n % 6 + 1) >= 0
+    for precision in ("fp64", "fp64_exact", "mixed"):
+        print(f"internal-energy branch {precision}", flush=True)
+        with hp.Box(st["p"].shape, precision) as box:
+            box.set_params(to_hp_params(P, hp.HpParams))
+            box.upload(copy_state(st))
+            for n in range(1, K + 1):
+                box.ppm_lxlylz(1 if n == 1 else n % 6 + 1)
+            out = box.download()
+        check(out, ref, precision, precision)
+
+
+def cold_cell(eint):
+    fl = [0.3, 0.3, 1.1, 1.1, 2.0, 2.0, 0.1, 0.1, -0.2, -0.2]
+    pres, rho, vx = 1e-4, 1.0, 2.0
+    return fl, [pres, rho, vx, 0.0, 0.0], pres * 1.5 + 0.5 * rho * vx * vx
+
+
+@pytest.mark.parametrize("eint", [3e-4, 7.5e-5, 0.0, -1.0])
+def test_integrate_cell_uses_callers_eint(oracle, eint):
+    """Equal face fluxes leave the caller's eint unchanged: in the e/E < eta1
+    regime pnew = (gamma-1) eint exactly, and eint <= 0 keeps the old p."""
+    P = oracle.params(ngrid=64, dt=0.01)
+    fl, st, etot = cold_cell(eint)
+    out = hp.integrate_cell(to_hp_params(P, hp.HpParams), fl, st, etot, eint, 0.0)
+    want = oracle.integrate(P, fl, st, etot, eint, 0.0)
+    assert np.array_equal(out, want), (out, want)
+    assert out[0] == ((P.gamma - 1) * eint if eint > 0 else st[0])
+
+
+def test_integrate_cell_through_the_fortran_wrapper(oracle):
+    """integrate_gpu, the Fortran drop-in for SUBROUTINE integrate, on the
+    same cold cell."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (no flang)")
+    eints = [3e-4, 7.5e-5, 0.0]
+    r = subprocess.run([DRIVER, "16", "0", "fp64", ".", "cells", "0"] + [repr(e) for e in eints],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = [float(l.split("pres=")[1]) for l in r.stdout.splitlines() if l.startswith("integrate:")]
+    want = [(5.0 / 3.0 - 1) * e if e > 0 else 1e-4 for e in eints]
+    assert got == want, (got, want)
+
+
+# ---- Newton non-convergence ---------------------------------------------------
+
+def newton_case(oracle):
+    """The 16^3 Sod tube's x sweep: 3 interfaces per pencil need 4 Newton
+    iterations, so a limit of 2 leaves 768 of them unconverged."""
+    st = ic.sod(16)
+    P = oracle.params(ngrid=16)
+    oracle.timedet(P, oracle.speed(P, st), 0.8, 1e30, 0.0)
+    oracle.set_newton_max(2)
+    try:
+        cnt = OracleCounters()
+        oracle.ppmsolver(P, copy_state(st), 1, counters=cnt)
+    finally:
+        oracle.set_newton_max(0)
+    return st, P, cnt.newton_nonconv
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64_exact", "mixed"])
+def test_newton_nonconvergence_returns_error(oracle, precision):
+    print(f"newton limit {precision}", flush=True)
+    st, P, want = newton_case(oracle)
+    assert want == 768
+    with hp.Box(st["p"].shape, precision) as box:
+        box.set_params(to_hp_params(P, hp.HpParams))
+        box.upload(copy_state(st))
+        box.ppmsolver(1)                                  # converges at the reference's limit
+        box.upload(copy_state(st))
+        box.set_newton_limit(2)
+        with pytest.raises(_capi.HydropadError) as e:
+            box.ppmsolver(1)
+        assert e.value.code == _capi.HP_E_NEWTON
+        assert box.counters().newton_nonconv == want
+        with pytest.raises(_capi.HydropadError) as e:     # the counter stays until reset
+            box.ppm_lxlylz(1)
+        assert e.value.code == _capi.HP_E_NEWTON
+        box.set_newton_limit(0)
+        box.reset_counters()
+        box.upload(copy_state(st))
+        box.ppm_lxlylz(1)
+        box.upload(copy_state(st))
+        box.set_newton_limit(2)
+        dt, t = box.timedet(box.speed(), 0.8)
+        with pytest.raises(_capi.HydropadError) as e:
+            box.evolve(1, 2, 0.8, t, dt)
+        assert e.value.code == _capi.HP_E_NEWTON
+
+
+def test_newton_nonconvergence_in_the_serving_loop():
+    reqs = [dict(n=16, nsteps=3, precision="fp64", seed=s, arrival_s=0.0) for s in (1, 2, 3)]
+    reqs[1]["newton_limit"] = 2
+    res, stats, _ = serve(reqs, hbm_budget=1 << 30, max_boxes=4)
+    assert [r["rc"] for r in res] == [0, _capi.HP_E_NEWTON, 0]
+    assert res[1]["newton_nonconv"] > 0 and res[0]["newton_nonconv"] == 0
+    assert stats["completed"] == 2
+
+
+def test_newton_stop_in_the_fortran_driver(tmp_path):
+    """The Fortran driver stops with the reference's message."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (no flang)")
+    r = subprocess.run([DRIVER, "16", "2", "fp64", str(tmp_path), "resident", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert "convergenza non raggiunta" in r.stdout, r.stdout + r.stderr
+    assert not (tmp_path / "rho.bin").exists()
