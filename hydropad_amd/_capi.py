@@ -75,7 +75,7 @@ EXPORTS = ("hp_params_init", "hp_create", "hp_destroy", "hp_set_precision", "hp_
            "hp_batch_box_bytes", "hp_batch_create", "hp_batch_destroy", "hp_batch_add", "hp_batch_add_sod",
            "hp_sod_of_seed", "hp_batch_remove", "hp_batch_set_newton_limit", "hp_batch_download", "hp_batch_step", "hp_batch_sync",
            "hp_batch_poll", "hp_batch_use_graph", "hp_batch_set_tuning", "hp_batch_get_tuning",
-           "hp_batch_bytes_used", "hp_batch_autotune", "hp_serve_run")
+           "hp_batch_bytes_used", "hp_batch_autotune", "hp_serve_run", "hp_ppmsolver_noalloc")
 
 _lib = None
 
@@ -124,6 +124,7 @@ def lib():
     L.hp_particles_download.argtypes = [_P, _P, _P]
     L.hp_kick.argtypes = [_P, _D]
     L.hp_drift.argtypes = [_P]
+    L.hp_ppmsolver_noalloc.argtypes = [PP] + [_P] * 11 + [_I] * 7 + [ctypes.POINTER(HpCounters), _P]
     _lib = L
     return L
 

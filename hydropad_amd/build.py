@@ -20,7 +20,7 @@ ARCH = os.environ.get("HP_OFFLOAD_ARCH", "gfx950")
 # order, so it is built without contraction; the throughput sweeps with it.
 SOURCES = [("hp_kernels.hip", "off"), ("hp_kernels_exact.hip", "off"), ("hp_kernels_fast.hip", "fast"),
            ("hp_capi.hip", "off"), ("hp_batch_kernels.hip", "fast"), ("hp_batch.hip", "off"),
-           ("hp_serve.hip", "off"), ("hp_nbody.hip", "off")]
+           ("hp_serve.hip", "off"), ("hp_noalloc.hip", "off"), ("hp_nbody.hip", "off")]
 
 
 def hipcc():

@@ -342,6 +342,30 @@ typedef struct hp_serve_stats {
 int hp_serve_run(const hp_request* reqs, int nreq, size_t hbm_budget, int max_boxes, int use_graph,
                  double* const* result_fields, hp_request_result* results, hp_serve_stats* stats);
 
+/* ---- one sweep with no device allocation -------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hp_ppmsolver's sweep (same kernels, same results) on
+ * caller-owned device buffers.  p..vz are the input fields and p_out..vz_out
+ * the output fields, (nx,ny,nz) REAL(8) device arrays in the reference layout
+ * (ghosts included; the output's face ghosts are written, edges and corners
+ * are not); input and output must not overlap.  gforce is a device
+ * (3,nx,ny,nz) array or NULL.  newton_limit as hp_set_newton_limit (0: 100).
+ * What hp_create would allocate besides the fields -- the device scalars and
+ * the status block -- is a library-owned array of HP_NOALLOC_SCRATCH_BYTES
+ * per GPU, held until the process ends and shared by every call on that GPU:
+ * calls that overlap on one GPU race (one stream at a time).  The call is
+ * stream-ordered on `stream` (a hipStream_t; NULL = the default stream) and
+ * calls no allocating routine.  counters == NULL: fully asynchronous;
+ * otherwise it synchronises the stream, fills *counters and returns
+ * HP_E_NEWTON on Newton non-convergence. */
+#define HP_NOALLOC_SCRATCH_BYTES 512
+int hp_ppmsolver_noalloc(const hp_params* prm, const double* p, const double* rho, const double* vx,
+                         const double* vy, const double* vz, double* p_out, double* rho_out, double* vx_out,
+                         double* vy_out, double* vz_out, const double* gforce, int nx, int ny, int nz,
+                         int nbound, int ndirection, int precision, int newton_limit, hp_counters* counters,
+                         void* stream);
+
 /* ---- scalar entry points (host-callable, for unit vectors) ---------------- */
 /* SUBROUTINE integrate (hydro/integrate.F90:1-113) on one cell, arguments by
  * pointer in the reference's order; pres, rho, vx, vy, vz are updated in place.
