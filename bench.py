@@ -150,10 +150,24 @@ def cpu_baseline(N, budget_s=10.0):
         O.set_threads(1)
         return N * ntr * ntr * nsteps / el, nsteps, el
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # every host thread this process may use: the box grants one GPU's share of
+    # the host and says so in OMP_NUM_THREADS; without it, every logical CPU
+    ncpu = os.cpu_count() or 1
+    try:
+        threads = max(1, min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu))))
+    except ValueError:
+        threads = ncpu
     v_all, n_all, el_all = timed(64, threads)
     v_one, n_one, el_one = timed(32, 1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     out = {"value": round(v_all, 1), "unit": "cell-updates/s", "cores": threads, "kind": "port",
+           "host_cpu": model, "host_logical_cpus": ncpu,
+           "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
            "sample": f"{N}x64x64 slab of the {N}^3 shock tube (dx=1/{N}), {n_all} steps "
                      f"(3 sweeps + periodic fill each) in {el_all:.1f} s on {threads} threads (OpenMP over pencils)",
            "single_thread": {"value": round(v_one, 1), "cores": 1,

@@ -76,3 +76,18 @@ hipError_t launch_batch_sweep(const BatchBox* boxes, const BatchPlan* plan, int 
                               int g53, int grid, hipStream_t s);
 
 }  // namespace hp
+
+struct hp_batch;
+struct hp_box_status;
+
+namespace hp {
+
+// Status reads that do not stall the stream (the serving loop's form of
+// hp_batch_poll): batch_status_async enqueues a copy of every box's status
+// and scalars into pinned host buffer `slot` (0 or 1) behind the work queued
+// so far and records an event; batch_status_wait waits for that event only
+// and decodes the copy, while later queued steps keep the GPU busy.
+int batch_status_async(hp_batch* b, int slot);
+int batch_status_wait(hp_batch* b, int slot, hp_box_status* out, int* nactive);
+
+}  // namespace hp

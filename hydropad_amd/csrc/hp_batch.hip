@@ -159,6 +159,11 @@ struct hp_batch {
     unsigned gcombos;
     int ggrid;
     std::vector<hp_box_status> h_status;
+    // pinned status copies for batch_status_async / batch_status_wait
+    hp::DevStatus* pin_st[2];
+    hp::Params* pin_prm[2];
+    hipEvent_t pin_ev[2];
+    int pin_n[2];
 };
 
 namespace {
@@ -404,6 +409,12 @@ int hp_batch_create(int max_boxes, size_t hbm_budget_bytes, hp_batch** out)
         delete b;
         return HP_E_HIP;
     }
+    for (int k = 0; k < 2; ++k) {
+        b->pin_st[k] = nullptr;
+        b->pin_prm[k] = nullptr;
+        b->pin_ev[k] = nullptr;
+        b->pin_n[k] = 0;
+    }
     b->nbox = 0;
     b->use_graph = false;
     b->gexec = nullptr;
@@ -418,6 +429,11 @@ void hp_batch_destroy(hp_batch* b)
     if (!b) return;
     hipStreamSynchronize(b->stream);
     drop_graph(b);
+    for (int k = 0; k < 2; ++k) {
+        if (b->pin_st[k]) hipHostFree(b->pin_st[k]);
+        if (b->pin_prm[k]) hipHostFree(b->pin_prm[k]);
+        if (b->pin_ev[k]) hipEventDestroy(b->pin_ev[k]);
+    }
     for (int i = 0; i < b->maxb; ++i)
         if (b->pools[i]) hipFree(b->pools[i]);
     hipFree(b->d_plan);                   // start of the descriptor block
@@ -499,20 +515,16 @@ int hp_batch_remove(hp_batch* b, int id)
     return push_nbox(b);
 }
 
-int hp_batch_poll(hp_batch* b, hp_box_status* out, int* nactive)
+// hp_box_status of every slot from copies of the first n device statuses
+// and scalars.
+static int decode_status(hp_batch* b, const hp::DevStatus* st, const hp::Params* prm, int n, hp_box_status* out,
+                         int* nactive)
 {
-    if (!b) return HP_E_ARG;
-    std::vector<hp::DevStatus> st(b->maxb);
-    if (b->nbox > 0)
-        HPCHK(hipMemcpyAsync(st.data(), b->d_st, sizeof(hp::DevStatus) * b->nbox, hipMemcpyDeviceToHost, b->stream));
-    HPCHK(hipStreamSynchronize(b->stream));
-    std::vector<hp::Params> prm(b->maxb);
-    if (b->nbox > 0) HPCHK(hipMemcpy(prm.data(), b->d_prm, sizeof(hp::Params) * b->nbox, hipMemcpyDeviceToHost));
     int na = 0;
     for (int i = 0; i < b->maxb; ++i) {
         hp_box_status s;
         memset(&s, 0, sizeof(s));
-        if (i < b->nbox && b->h_boxes[i].active) {
+        if (i < n && b->h_boxes[i].active) {
             s.active = 1;
             s.done = st[i].done;
             s.steps_done = st[i].nstep_done;
@@ -529,6 +541,21 @@ int hp_batch_poll(hp_batch* b, hp_box_status* out, int* nactive)
     if (nactive) *nactive = na;
     return HP_OK;
 }
+
+int hp_batch_poll(hp_batch* b, hp_box_status* out, int* nactive)
+{
+    if (!b) return HP_E_ARG;
+    std::vector<hp::DevStatus> st(b->maxb);
+    std::vector<hp::Params> prm(b->maxb);
+    if (b->nbox > 0) {
+        HPCHK(hipMemcpyAsync(st.data(), b->d_st, sizeof(hp::DevStatus) * b->nbox, hipMemcpyDeviceToHost, b->stream));
+        HPCHK(hipMemcpyAsync(prm.data(), b->d_prm, sizeof(hp::Params) * b->nbox, hipMemcpyDeviceToHost, b->stream));
+    }
+    HPCHK(hipStreamSynchronize(b->stream));
+    return decode_status(b, st.data(), prm.data(), b->nbox, out, nactive);
+}
+
+
 
 int hp_batch_download(hp_batch* b, int id, double* p, double* rho, double* vx, double* vy, double* vz)
 {
@@ -683,3 +710,36 @@ int hp_batch_autotune(hp_batch* b, int steps_per_trial, hp_batch_tuning* best, d
 }
 
 }  // extern "C"
+
+namespace hp {
+
+int batch_status_async(hp_batch* b, int slot)
+{
+    if (!b || slot < 0 || slot > 1) return HP_E_ARG;
+    if (!b->pin_st[slot]) {
+        if (hipHostMalloc((void**)&b->pin_st[slot], sizeof(hp::DevStatus) * b->maxb, hipHostMallocDefault) !=
+                hipSuccess ||
+            hipHostMalloc((void**)&b->pin_prm[slot], sizeof(hp::Params) * b->maxb, hipHostMallocDefault) !=
+                hipSuccess ||
+            hipEventCreateWithFlags(&b->pin_ev[slot], hipEventDisableTiming) != hipSuccess)
+            return HP_E_NOMEM;
+    }
+    const int n = b->nbox;
+    b->pin_n[slot] = n;
+    if (n > 0) {
+        HPCHK(hipMemcpyAsync(b->pin_st[slot], b->d_st, sizeof(hp::DevStatus) * n, hipMemcpyDeviceToHost, b->stream));
+        HPCHK(hipMemcpyAsync(b->pin_prm[slot], b->d_prm, sizeof(hp::Params) * n, hipMemcpyDeviceToHost, b->stream));
+    }
+    HPCHK(hipEventRecord(b->pin_ev[slot], b->stream));
+    return HP_OK;
+}
+
+int batch_status_wait(hp_batch* b, int slot, hp_box_status* out, int* nactive)
+{
+    if (!b || slot < 0 || slot > 1 || !b->pin_ev[slot]) return HP_E_ARG;
+    HPCHK(hipEventSynchronize(b->pin_ev[slot]));
+    return decode_status(b, b->pin_st[slot], b->pin_prm[slot], b->pin_n[slot], out, nactive);
+}
+
+}  // namespace hp
+

@@ -259,14 +259,10 @@ template <class R> __device__ __forceinline__ Parab<R> limit_parab_d(R l, R r, R
     return q;
 }
 
-// C*beta (ppm_functs.F90:129-144: beta = -sign sum / (2C)) on offsets: ut-uc
-// and pt-pc are offset differences; pc/uc/gc absolute values enter only the
-// expansion/gravity terms.  The face assembly below works with these
-// amplitudes (velocity units) instead of beta itself, which grows as 1/C^2:
-// with a clamped traced state (p or rho = 1e-15) beta's magnitude leaves
-// FP32's range, C*beta's does not.
+// beta (ppm_functs.F90:129-144) on offsets: ut-uc and pt-pc are offset
+// differences; pc/uc/gc absolute values enter only the expansion/gravity terms.
 template <class R, bool GRAV, bool COSMO>
-__device__ __forceinline__ R ampl_d(const SweepScalars<R>& S, R rct, R ut, R uc, R pt, R pc, R pc_abs, R uc_abs,
+__device__ __forceinline__ R beta_d(const SweepScalars<R>& S, R rct, R ut, R uc, R pt, R pc, R pc_abs, R uc_abs,
                                     R gc_abs, R sign)
 {
     R sum = (ut - uc) + sign * (pt - pc) * rct;
@@ -275,7 +271,7 @@ __device__ __forceinline__ R ampl_d(const SweepScalars<R>& S, R rct, R ut, R uc,
         R gvc = ((GRAV ? -gc_abs : R(0)) + (COSMO ? S.dat * uc_abs : R(0))) * S.rat;
         sum = sum + S.dt * gvc;
     }
-    return -sign * sum * R(0.5);
+    return -sign * sum * R(0.5) * rct;
 }
 
 // Traced face state of a cell.  RIGHT = true: its right face (the
@@ -317,38 +313,36 @@ __device__ __forceinline__ Face<double> trace_face_d(const SweepScalars<R>& S, c
     const R ct = fsqrt((R)(gamma * ptl * rtl));
     const R rct = rcp(ct);
     const R ap = (R)a[VP], au = (R)a[VU];
-    R am_ = R(0), ap_ = R(0), x0 = R(0);        // C*beta-, C*beta+, rho_t*beta0
+    R bm = R(0), bp = R(0), b0 = R(0);
     R vf = vt, wf = wt;
     if (RIGHT) {
-        if (um > 0.0) am_ = ampl_d<R, GRAV, COSMO>(S, rct, ut, uc1, pt, pc1, ap + pc1, au + uc1, gc1, R(-1));
-        if (up > 0.0) ap_ = ampl_d<R, GRAV, COSMO>(S, rct, ut, uc3, pt, pc3, ap + pc3, au + uc3, gc3, R(1));
+        if (um > 0.0) bm = beta_d<R, GRAV, COSMO>(S, rct, ut, uc1, pt, pc1, ap + pc1, au + uc1, gc1, R(-1));
+        if (up > 0.0) bp = beta_d<R, GRAV, COSMO>(S, rct, ut, uc3, pt, pc3, ap + pc3, au + uc3, gc3, R(1));
         if (u0 > 0.0) {
             vf = vc2;
             wf = wc2;
         }
     } else {
-        if (up < 0.0) ap_ = ampl_d<R, GRAV, COSMO>(S, rct, ut, uc3, pt, pc3, ap + pc3, au + uc3, gc3, R(1));
-        if (um < 0.0) am_ = ampl_d<R, GRAV, COSMO>(S, rct, ut, uc1, pt, pc1, ap + pc1, au + uc1, gc1, R(-1));
+        if (up < 0.0) bp = beta_d<R, GRAV, COSMO>(S, rct, ut, uc3, pt, pc3, ap + pc3, au + uc3, gc3, R(1));
+        if (um < 0.0) bm = beta_d<R, GRAV, COSMO>(S, rct, ut, uc1, pt, pc1, ap + pc1, au + uc1, gc1, R(-1));
         if (u0 < 0.0) {
             vf = vc2;
             wf = wc2;
         }
     }
-    const R rtr = (R)rtl;
     if (RIGHT ? (u0 > 0.0) : (u0 < 0.0)) {
-        // rho_t * beta0 (ppm_functs.F90:149-162): (p_t - p_c) rho_t/C^2 + (rho_c - rho_t)/rho_c,
-        // with rho_t/C^2 = 1/(gamma p_t) formed as (rho_t/C)/C
-        const R rgp = (rct * rtr) * rct;
-        const R rcr = (R)a[VR] + rc2;
-        x0 = (pt - pc2) * rgp + div(rc2 - rt, rcr);
-        if (COSMO) x0 = x0 + S.cdat2 * (ap + pc2) * rgp;                 // 2 adot pc dt/(a C^2) * rho_t
+        // beta0 (ppm_functs.F90:149-162) with 1/rho_t - 1/rho_c = (rho_c - rho_t)/(rho_t rho_c)
+        const R rct2 = rct * rct;
+        const R rtr = (R)rtl, rcr = (R)a[VR] + rc2;
+        b0 = (pt - pc2) * rct2 + div(rc2 - rt, rtr * rcr);
+        if (COSMO) b0 = b0 + S.cdat2 * (ap + pc2) * rct2;                 // 2 adot pc dt/(a C^2)
     }
     Face<double> f;
-    f.p = ptl + (double)(ct * (ap_ + am_));                               // + C^2 (beta+ + beta-)
+    f.p = ptl + (double)(ct * ct * (bp + bm));
     if (f.p <= 0.0) f.p = ptl;
-    f.u = a[VU] + (double)(ut + (ap_ - am_));                             // + C (beta+ - beta-)
+    f.u = a[VU] + (double)(ut + ct * (bp - bm));
     // 1/(1/rho_t - bm - b0 - bp) = rho_t / (1 - rho_t (bm + b0 + bp))
-    f.r = rtl * rcp(1.0 - (double)(rtr * rct * (am_ + ap_) + x0));
+    f.r = rtl * rcp(1.0 - (double)((R)rtl * (bm + b0 + bp)));
     if (f.r <= 0.0) f.r = rtl;
     f.v = a[VV] + (double)vf;
     f.w = a[VW] + (double)wf;

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/hydropad_gpu.h"
+#include "hp_batch.h"
 
 namespace {
 
@@ -53,6 +54,14 @@ extern "C" int hp_serve_run(const hp_request* reqs, int nreq, size_t hbm_budget,
     }
     std::vector<int> owner(max_boxes, -1);
     std::vector<hp_box_status> st(max_boxes);
+    // The step loop keeps one step queued ahead: step n+1 is enqueued, with an
+    // asynchronous copy of the statuses behind it, before the host waits for
+    // step n's copy, so the GPU never idles on a per-step synchronisation.
+    // owner_at[k] is the slot -> request map when status copy k was queued: a
+    // slot freed and re-admitted since then is not read from that copy.
+    std::vector<int> owner_at[2] = {std::vector<int>(max_boxes, -1), std::vector<int>(max_boxes, -1)};
+    int cslot = 0;
+    bool inflight = false;
     const auto t0 = std::chrono::steady_clock::now();
     auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
     int next = 0, completed = 0, nactive = 0;
@@ -102,11 +111,19 @@ extern "C" int hp_serve_run(const hp_request* reqs, int nreq, size_t hbm_budget,
         rc = hp_batch_step(b, 1);
         if (rc) break;
         ++steps;
-        rc = hp_batch_poll(b, st.data(), nullptr);
+        rc = hp::batch_status_async(b, cslot);
+        if (rc) break;
+        owner_at[cslot] = owner;
+        const bool have_prev = inflight;
+        const int pslot = cslot ^ 1;
+        inflight = true;
+        cslot = pslot;
+        if (!have_prev) continue;                       // nothing to read yet: queue the next step first
+        rc = hp::batch_status_wait(b, pslot, st.data(), nullptr);
         if (rc) break;
         const double tdone = now();
         for (int id = 0; id < max_boxes; ++id) {
-            if (owner[id] < 0 || !st[id].done) continue;
+            if (owner[id] < 0 || owner_at[pslot][id] != owner[id] || !st[id].done) continue;
             const int i = owner[id];
             const hp_request& r = reqs[i];
             results[i].done_s = tdone;

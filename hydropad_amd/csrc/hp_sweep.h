@@ -593,7 +593,6 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
     double iv2[NV];                          // interface value at k-2 (left of cell k-2)
     unsigned bits = 0;                       // pair bits of k-3, k-2, k-1
     Face<double> fRprev;                     // right face of cell k-3
-    int cRprev = 0;                          // its clamp count
     Flux Fprev;                              // flux at interface k-3 (left of cell k-3)
 #pragma unroll
     for (int s = 0; s < NV; ++s) { w0[s] = w1[s] = w2[s] = w3[s] = iv2[s] = 0.0; d1[s] = d2[s] = R(0); }
@@ -638,23 +637,24 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
             for (int s = 0; s < NV; ++s) iv1[s] = M::iface(w2[s], w3[s], d2[s], d3[s]);
 
             Face<double> fL = fRprev, fR = fRprev;
-            int cL = 0, cR = 0;
+            const int ic = kk - 2;                                   // cell k-2, interface at its left
             if (kk >= s0 + 1) {     // parabola and faces of cell k-2
+                // clamp count: both faces of the cells s0-1 .. s1 a segment
+                // reconstructs (per interface only in the x sweep: an exact
+                // count here costs the FP64 form half its occupancy)
                 const bool flat = flat_always || bits != 0u || bk;
                 M::template cell_faces<R, NV, GRAV, COSMO>(A, w2, iv2, iv1, w0[VR], w1[VR], w3[VR], cur[VR], d1[VR],
-                                                           d3[VR], w1[VP], w3[VP], flat, fL, fR, cL, cR);
+                                                           d3[VR], w1[VP], w3[VP], flat, fL, fR, T.nclamp, T.nclamp);
             }
             Flux F = Fprev;
             if (kk >= s0 + 2) {     // Riemann problem at interface k-2
                 int nit;
                 F = M::riemann(A.Rs, fRprev, fL, nit);
                 peak_floor(A.Rs, F, r5, w0[VR], w1[VR], w2[VR], w0[VU], w1[VU]);
-                // counted once per interface of the pencil (0 .. Nax): a
-                // segment owns interfaces s0 .. s1-1, the last one also Nax
-                if (kk - 2 < s1 || s1 == Nax) {
-                    count_newton(nit, T.nitmax, T.nonconv);
-                    T.nclamp += cRprev + cL;
-                }
+                // Newton count once per interface of the pencil (0 .. Nax), as
+                // the reference's loop over interfaces: a segment owns
+                // interfaces s0 .. s1-1 and the pencil's last segment also Nax
+                if (kk != s1 + 2 || s1 == Nax) count_newton(nit, T.nitmax, T.nonconv);
             }
             if (kk >= s0 + 3) {     // update cell k-3
                 double pres = w1[VP], rho = w1[VR], u = w1[VU], vv = w1[VV], w = w1[VW];
@@ -677,7 +677,6 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
             }
             bits = (bits >> 1) | ((bk ? 1u : 0u) << 2);
             fRprev = fR;
-            cRprev = cR;
             Fprev = F;
         }
     }
