@@ -427,23 +427,29 @@ __device__ __forceinline__ void newton_side(const RScal& Rs, bool shock, double 
     }
 }
 
-// van Leer Newton iteration in FP64 (riemann.F90:58-109) from the initial
-// guess in pstar; the same iterates as the reference up to rounding.  vstar is
-// the reference's (pl - pr + a vl + b vr)/(a + b) with the last iteration's
-// impedances, written with 1/a and 1/b.
+// van Leer Newton iteration in FP64 (riemann.F90:58-109) from the iterate in
+// pstar, with nnewt iterations already done; the same iterates as the
+// reference up to rounding.  Runs until the relative change is within terr
+// (returns 1), until kmax iterations in all (returns 0: not finished; call
+// again to continue with the same iterate sequence), or past the reference's
+// limit (returns -1 with nnewt = -1, where riemann.F90:60-65 stops).  vstar
+// (returns 1 and -1) is the reference's (pl - pr + a vl + b vr)/(a + b) with
+// the last iteration's impedances, written with 1/a and 1/b.
 // cleft/cright are the Lagrangian sound speeds sqrt(gamma p rho) of the two
 // sides (riemann.F90:68-69), rcl/rcr their inverses.
-__device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vstar, double pleft, double pright,
-                                      double vxleft, double vxright, double cleft, double rcl, double cright,
-                                      double rcr)
+__device__ __forceinline__ int newton_resume(const RScal& Rs, double& pstar, double& vstar, int& nnewt, int kmax,
+                                             double pleft, double pright, double vxleft, double vxright,
+                                             double cleft, double rcl, double cright, double rcr)
 {
     const double cl2 = cleft * cleft, cr2 = cright * cright;
     const double rpl = rcp(pleft), rpr = rcp(pright);
     double ra = 0.0, rb = 0.0;
-    int nnewt = 0;
+    int st = 1;
     for (;;) {
+        if (nnewt >= kmax) return 0;
         if (++nnewt > Rs.nmax) {                     // where the reference stops (riemann.F90:60-65)
             nnewt = -1;
+            st = -1;
             break;
         }
         const double pstarold = pstar;
@@ -458,6 +464,16 @@ __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vs
         if (!(fabs(pstarold - pstar) > K_TERR * pstarold)) break;
     }
     vstar = div(fma((pleft - pright) * ra, rb, fma(vxleft, rb, vxright * ra)), ra + rb);
+    return st;
+}
+
+// The whole iteration from the initial guess: iterations done, or -1.
+__device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vstar, double pleft, double pright,
+                                      double vxleft, double vxright, double cleft, double rcl, double cright,
+                                      double rcr)
+{
+    int nnewt = 0;
+    newton_resume(Rs, pstar, vstar, nnewt, 0x7fffffff, pleft, pright, vxleft, vxright, cleft, rcl, cright, rcr);
     return nnewt;
 }
 
@@ -477,23 +493,31 @@ template <class R> __device__ __forceinline__ R vshock(R rs, R vxm, R rho, R vx)
     return (rs == rho) ? R(0) : div(rs * vxm - rho * vx, rs - rho);
 }
 
-// riemannsolver.F90:32-285, FP64 throughout (the mixed form keeps the Riemann
-// problem in FP64: its star state feeds the fluxes directly).
-__device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double>& L, const Face<double>& Rt, int& nit)
+// A Riemann problem after the wave-pattern classification of
+// riemannsolver.F90:32-64: Eulerian sound speeds and inverses, the pattern,
+// and for patterns 1 and 2 the Newton iterate (initial guess, then result).
+struct RSolve {
+    double rrl, rrr, rcle, cleft, rcre, cright;
+    int kase, nit;
+    double pm, vxm;
+};
+
+// Classification and initial Newton guess (riemannsolver.F90:32-64,77,117-119,
+// riemann.F90:54-56).
+__device__ __forceinline__ void riemann_setup(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
+                                              RSolve& S)
 {
     using R = double;
-    const R gamma = (R)Rs.gamma, m = (R)Rs.m, rm = (R)Rs.rm, rgamma = (R)Rs.rgamma, rgamma1 = (R)Rs.rgamma1;
-    const R g1 = (R)Rs.gamma1;
+    const R gamma = (R)Rs.gamma, rgamma1 = (R)Rs.rgamma1;
     const R pleft = L.p, pright = Rt.p, rholeft = L.r, rhoright = Rt.r;
     const R vxleft = L.u, vxright = Rt.u;
-    R pm = R(0), rhom = R(0), vxm = R(0);
-    nit = 0;
     // Eulerian sound speeds and their inverses from one rsq each; the
     // Newton iteration's Lagrangian C = rho c and 1/C = (1/c)(1/rho) follow
-    const double rrl = rcp(rholeft), rrr = rcp(rhoright);
-    double rcle, cleft, rcre, cright;
-    rsq_sqrt(gamma * pleft * rrl, rcle, cleft);
-    rsq_sqrt(gamma * pright * rrr, rcre, cright);
+    S.rrl = rcp(rholeft);
+    S.rrr = rcp(rhoright);
+    rsq_sqrt(gamma * pleft * S.rrl, S.rcle, S.cleft);
+    rsq_sqrt(gamma * pright * S.rrr, S.rcre, S.cright);
+    const double cleft = S.cleft, cright = S.cright;
     R p1, p2, rho1, c1, c2;
     if (pleft <= pright) { p1 = pleft; p2 = pright; rho1 = rholeft; c1 = cleft; c2 = cright; }
     else { p1 = pright; p2 = pleft; rho1 = rhoright; c1 = cright; c2 = cleft; }
@@ -508,21 +532,112 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
         const R dv1s = -(R(2) * c2 * rgamma1) * (R(1) - pow_m(Rs, div(p1, p2)));
         kase = (dvv >= dv1s) ? 2 : (dvv > dv2r ? 3 : 4);
     }
+    S.kase = kase;
+    S.nit = 0;
+    S.pm = 0.0;
+    S.vxm = 0.0;
     if (kase == 1 || kase == 2) {
-        double pmd = 0.0, vxmd = 0.0;
+        double pmd = 0.0;
         if (!fm::pcheck_guess(Rs, pleft, pright, rholeft, rhoright, vxleft, vxright, pmd)) {
             if (kase == 1) pmd = (double)tmax(pleft, pright);        // riemannsolver.F90:77
             else                                                      // :117-119
                 pmd = pow_rm(Rs, div(cleft + cright + (vxleft - vxright) * (gamma - R(1)) * R(0.5),
                                      div(cleft, pow_m(Rs, pleft)) + div(cright, pow_m(Rs, pright))));
         }
-        nit = fm::newton(Rs, pmd, vxmd, pleft, pright, vxleft, vxright, rholeft * cleft, rcle * rrl,
-                         rhoright * cright, rcre * rrr);
-        pm = (R)pmd;
-        vxm = (R)vxmd;
+        S.pm = pmd;
+    }
+}
+
+// Newton iterations of patterns 1 and 2 from S.pm (newton_resume's contract).
+__device__ __forceinline__ int riemann_newton(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
+                                              RSolve& S, int kmax)
+{
+    return newton_resume(Rs, S.pm, S.vxm, S.nit, kmax, L.p, Rt.p, L.u, Rt.u, L.r * S.cleft, S.rcle * S.rrl,
+                         Rt.r * S.cright, S.rcre * S.rrr);
+}
+
+// The costly sampling operations in precision Q.  Q = double: the FP64
+// form's (the reference's algebra, divisions and roots to within an ulp).
+// Q = float: the mixed form, which takes the shock density, the star-region
+// sound speed and the fan interpolation weight in FP32 (v_rcp_f32,
+// v_sqrt_f32).  Each of those FP32 results scales a difference (the density
+// jump across the shock, the distance of a wave speed from 0, f1 - f2) or
+// only steers a switch, so its rounding stays at FP32 relative size of that
+// difference.  The rarefaction density rho (pm/p)^(1/gamma) and the
+// two-rarefaction star state stay in FP64: FP32 powers there carry FP32
+// rounding of the full state, and over 300 steps of the 512^3 Sod tube that
+// raised the mixed form's L1(rho) distance to the FP64 form from 1e-7 to 2e-5.
+template <class Q> struct SampleOps;
+template <> struct SampleOps<double> {
+    static __device__ __forceinline__ double shock_density(double g1, double pm, double p, double rho)
+    {
+        return rhoshock(g1, pm, p, rho);
+    }
+    static __device__ __forceinline__ double fan_density(const RScal& Rs, double pm, double p, double rho)
+    {
+        return rho * pow_rgamma(Rs, div(pm, p));
+    }
+    static __device__ __forceinline__ double sound(double gamma, double p, double rho)
+    {
+        return fsqrt(gamma * p * rcp(rho));
+    }
+    static __device__ __forceinline__ double fan(double f1, double f2, double vt, double vh)
+    {
+        return fm::rarwave(f1, f2, vt, vh);
+    }
+    static __device__ __forceinline__ double ratio(double a, double b) { return div(a, b); }
+    static __device__ __forceinline__ double powm_ratio(const RScal& Rs, double a, double b)
+    {
+        return pow_m(Rs, div(a, b));
+    }
+};
+template <> struct SampleOps<float> : SampleOps<double> {
+    // Rankine-Hugoniot (riemann.F90:22) in delta form: rho (g1 x + 1)/(g1 + x)
+    // = rho + rho (g1 - 1)(pm - p)/(g1 p + pm), the FP32 ratio scaling the
+    // jump only (exactly rho when pm = p)
+    static __device__ __forceinline__ double shock_density(double g1, double pm, double p, double rho)
+    {
+        const float num = (float)((g1 - 1.0) * (pm - p));
+        const float den = (float)(g1 * p + pm);
+        return rho + rho * (double)(num * __builtin_amdgcn_rcpf(den));
+    }
+    static __device__ __forceinline__ double sound(double gamma, double p, double rho)
+    {
+        return (double)__builtin_amdgcn_sqrtf((float)(gamma * p * __builtin_amdgcn_rcp(rho)));
+    }
+    // riemann.F90:7: f2 + (f1 - f2) vt/(vt - vh), the weight in FP32
+    static __device__ __forceinline__ double fan(double f1, double f2, double vt, double vh)
+    {
+        return f2 + (double)(float)(vt * __builtin_amdgcn_rcp(vt - vh)) * (f1 - f2);
+    }
+};
+
+// Sampling at x/t = 0 (riemannsolver.F90:73-275) from the classified problem
+// and, for patterns 1 and 2, the Newton result; the costly operations in
+// precision Q (SampleOps), switch decisions in FP64.
+template <class Q>
+__device__ __forceinline__ Flux riemann_sample(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
+                                               const RSolve& S)
+{
+    using R = double;
+    using O = SampleOps<Q>;
+    const R gamma = (R)Rs.gamma, m = (R)Rs.m, rm = (R)Rs.rm, rgamma = (R)Rs.rgamma, rgamma1 = (R)Rs.rgamma1;
+    const R g1 = (R)Rs.gamma1;
+    (void)m; (void)rm; (void)rgamma;
+    const R pleft = L.p, pright = Rt.p, rholeft = L.r, rhoright = Rt.r;
+    const R vxleft = L.u, vxright = Rt.u;
+    const double cleft = S.cleft, cright = S.cright, rcle = S.rcle;
+    const int kase = S.kase;
+    R pm = R(0), rhom = R(0), vxm = R(0);
+    if (kase == 0) {                                      // contact at rest (riemann_solve)
+        pm = pleft;
+        rhom = rhoright;
+    } else if (kase == 1 || kase == 2) {
+        pm = (R)S.pm;
+        vxm = (R)S.vxm;
         if (kase == 1) {
-            const R rs1 = rhoshock(g1, pm, pleft, rholeft);
-            const R rs2 = rhoshock(g1, pm, pright, rhoright);
+            const R rs1 = O::shock_density(g1, pm, pleft, rholeft);
+            const R rs2 = O::shock_density(g1, pm, pright, rhoright);
             const R vs1 = vshock(rs1, vxm, rholeft, vxleft);
             const R vs2 = vshock(rs2, vxm, rhoright, vxright);
             if (vs2 < R(0)) { pm = pright; vxm = vxright; rhom = rhoright; }
@@ -530,60 +645,60 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
             else if (vs1 <= R(0) && vxm > R(0)) rhom = rs1;
             else { pm = pleft; vxm = vxleft; rhom = rholeft; }
         } else if (pleft >= pright) {
-            const R rs2 = rhoshock(g1, pm, pright, rhoright);
-            const R rc1 = rholeft * pow_rgamma(Rs, div(pm, pleft));
+            const R rs2 = O::shock_density(g1, pm, pright, rhoright);
+            const R rc1 = O::fan_density(Rs, pm, pleft, rholeft);
             const R vs2 = vshock(rs2, vxm, rhoright, vxright);
-            const R vtail1 = vxm - fsqrt(gamma * pm * rcp(rc1));
+            const R vtail1 = vxm - O::sound(gamma, pm, rc1);
             const R vhead1 = vxleft - cleft;
             if (vs2 < R(0)) { pm = pright; vxm = vxright; rhom = rhoright; }
             else if (vhead1 >= R(0)) { pm = pleft; vxm = vxleft; rhom = rholeft; }
             else if (vxm <= R(0) && vs2 >= R(0)) rhom = rs2;
             else if (vxm > R(0) && vtail1 <= R(0)) rhom = rc1;
             else if (vtail1 > R(0) && vhead1 <= R(0)) {
-                pm = fm::rarwave(pleft, pm, vtail1, vhead1);
-                rhom = fm::rarwave(rholeft, rc1, vtail1, vhead1);
-                vxm = fm::rarwave(vxleft, vxm, vtail1, vhead1);
+                pm = O::fan(pleft, pm, vtail1, vhead1);
+                rhom = O::fan(rholeft, rc1, vtail1, vhead1);
+                vxm = O::fan(vxleft, vxm, vtail1, vhead1);
             }
         } else {
-            const R rs1 = rhoshock(g1, pm, pleft, rholeft);
-            const R rc2 = rhoright * pow_rgamma(Rs, div(pm, pright));
+            const R rs1 = O::shock_density(g1, pm, pleft, rholeft);
+            const R rc2 = O::fan_density(Rs, pm, pright, rhoright);
             const R vs1 = vshock(rs1, vxm, rholeft, vxleft);
-            const R vtail2 = vxm + fsqrt(gamma * pm * rcp(rc2));
+            const R vtail2 = vxm + O::sound(gamma, pm, rc2);
             const R vhead2 = vxright + cright;
             if (vs1 > R(0)) { pm = pleft; vxm = vxleft; rhom = rholeft; }
             else if (vhead2 <= R(0)) { pm = pright; vxm = vxright; rhom = rhoright; }
             else if (vxm >= R(0) && vs1 <= R(0)) rhom = rs1;
             else if (vxm < R(0) && vtail2 >= R(0)) rhom = rc2;
             else if (vtail2 < R(0) && vhead2 >= R(0)) {
-                pm = fm::rarwave(pright, pm, vtail2, vhead2);
-                rhom = fm::rarwave(rhoright, rc2, vtail2, vhead2);
-                vxm = fm::rarwave(vxright, vxm, vtail2, vhead2);
+                pm = O::fan(pright, pm, vtail2, vhead2);
+                rhom = O::fan(rhoright, rc2, vtail2, vhead2);
+                vxm = O::fan(vxright, vxm, vtail2, vhead2);
             }
         }
     } else if (kase == 3) {                               // two rarefactions (no Newton)
-        const R zeta = cright * rcle * pow_m(Rs, div(pleft, pright));
+        const R zeta = cright * rcle * O::powm_ratio(Rs, pleft, pright);
         const R vtleft = vxleft + R(2) * cleft * rgamma1;
         const R vtright = vxright - R(2) * cright * rgamma1;
-        vxm = div(vtleft * zeta + vtright, R(1) + zeta);
-        pm = pleft * pow_rm(Rs, div((gamma - R(1)) * (vtleft - vtright), R(2) * cleft * (R(1) + zeta)));
-        const R rc1 = rholeft * pow_rgamma(Rs, div(pm, pleft));
-        const R rc2 = rhoright * pow_rgamma(Rs, div(pm, pright));
-        const R vtail2 = vxm + fsqrt(gamma * pm * rcp(rc2));
+        vxm = O::ratio(vtleft * zeta + vtright, R(1) + zeta);
+        pm = pleft * pow_rm(Rs, O::ratio((gamma - R(1)) * (vtleft - vtright), R(2) * cleft * (R(1) + zeta)));
+        const R rc1 = O::fan_density(Rs, pm, pleft, rholeft);
+        const R rc2 = O::fan_density(Rs, pm, pright, rhoright);
+        const R vtail2 = vxm + O::sound(gamma, pm, rc2);
         const R vhead2 = vxright + cright;
-        const R vtail1 = vxm - fsqrt(gamma * pm * rcp(rc1));
+        const R vtail1 = vxm - O::sound(gamma, pm, rc1);
         const R vhead1 = vxleft - cleft;
         if (vhead2 <= R(0)) { pm = pright; vxm = vxright; rhom = rhoright; }
         else if (vhead1 >= R(0)) { pm = pleft; vxm = vxleft; rhom = rholeft; }
         else if (vhead2 > R(0) && vtail2 <= R(0)) {
-            pm = fm::rarwave(pright, pm, vtail2, vhead2);
-            rhom = fm::rarwave(rhoright, rc2, vtail2, vhead2);
-            vxm = fm::rarwave(vxright, vxm, vtail2, vhead2);
+            pm = O::fan(pright, pm, vtail2, vhead2);
+            rhom = O::fan(rhoright, rc2, vtail2, vhead2);
+            vxm = O::fan(vxright, vxm, vtail2, vhead2);
         } else if (vxm <= R(0) && vtail2 > R(0)) rhom = rc2;
         else if (vxm > R(0) && vtail1 <= R(0)) rhom = rc1;
         else if (vtail1 > R(0) && vhead1 <= R(0)) {
-            pm = fm::rarwave(pleft, pm, vtail1, vhead1);
-            rhom = fm::rarwave(rholeft, rc1, vtail1, vhead1);
-            vxm = fm::rarwave(vxleft, vxm, vtail1, vhead1);
+            pm = O::fan(pleft, pm, vtail1, vhead1);
+            rhom = O::fan(rholeft, rc1, vtail1, vhead1);
+            vxm = O::fan(vxleft, vxm, vtail1, vhead1);
         }
     } else {                                              // vacuum
         pm = R(0);
@@ -595,14 +710,14 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
         if (vhead2 <= R(0)) { pm = pright; vxm = vxright; rhom = rhoright; }
         else if (vhead1 >= R(0)) { pm = pleft; vxm = vxleft; rhom = rholeft; }
         else if (vhead2 > R(0) && vtail2 <= R(0)) {
-            pm = fm::rarwave(pright, pm, vtail2, vhead2);
-            rhom = fm::rarwave(rhoright, R(0), vtail2, vhead2);
-            vxm = fm::rarwave(vxright, vxm, vtail2, vhead2);
+            pm = O::fan(pright, pm, vtail2, vhead2);
+            rhom = O::fan(rhoright, R(0), vtail2, vhead2);
+            vxm = O::fan(vxright, vxm, vtail2, vhead2);
         } else if (vtail1 <= R(0) && vtail2 > R(0)) rhom = R(0);
         else if (vtail1 > R(0) && vhead1 <= R(0)) {
-            pm = fm::rarwave(pleft, pm, vtail1, vhead1);
-            rhom = fm::rarwave(rholeft, R(0), vtail1, vhead1);
-            vxm = fm::rarwave(vxleft, vxm, vtail1, vhead1);
+            pm = O::fan(pleft, pm, vtail1, vhead1);
+            rhom = O::fan(rholeft, R(0), vtail1, vhead1);
+            vxm = O::fan(vxleft, vxm, vtail1, vhead1);
         }
     }
     Flux F;
@@ -612,6 +727,35 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
     if (vxm >= R(0)) { F.v = Rs.dadt * (double)L.v; F.w = Rs.dadt * (double)L.w; }
     else { F.v = Rs.dadt * (double)Rt.v; F.w = Rs.dadt * (double)Rt.w; }
     return F;
+}
+
+// riemannsolver.F90:32-285: classification and Newton iteration in FP64, the
+// sampling's costly operations in precision Q (FP64 form: double; mixed
+// form: float).
+template <class Q = double>
+__device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double>& L, const Face<double>& Rt, int& nit)
+{
+    // A contact at rest (equal pressures, no normal velocity on either side)
+    // is a zero-strength problem: the two-shock branch, its initial guess
+    // pcheck = p and one Newton iteration return p* = p and v* = 0 exactly in
+    // this arithmetic (p/p, (g1+1)/(g1+1) and 0/(a+b) are exact), and the
+    // sampling picks the right state's density.  The transverse sweeps of a
+    // planar flow are made of these.  (Not in the reference-order form, whose
+    // rhoshock rounds rho (g1+1)/(g1+1) and can differ from rho by an ulp.)
+    // The mixed form's FP32 sampling returns the same state from the full
+    // path (its shock density is in delta form, exact for pm = p).  Routed
+    // through the sampling as pattern 0 so the transverse-velocity upwinding
+    // stays one selection of values.
+    RSolve S;
+    if (L.p == Rt.p && L.u == 0.0 && Rt.u == 0.0) {
+        S.kase = 0;
+        S.nit = 1;
+    } else {
+        riemann_setup(Rs, L, Rt, S);
+        if (S.kase == 1 || S.kase == 2) riemann_newton(Rs, L, Rt, S, 0x7fffffff);
+    }
+    nit = S.nit;
+    return riemann_sample<Q>(Rs, L, Rt, S);
 }
 
 // ---- conservative update + dual energy (FP64) ------------------------------------------

@@ -48,6 +48,7 @@ struct ExactMath {
                                                       const double* ivr, double rm2, double rm1, double rp1,
                                                       double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
                                                       Face<double>& fL, Face<double>& fR, int& nclL, int& nclR);
+    template <class R>
     static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
                                                    int& nit)
     {
@@ -79,10 +80,12 @@ struct FastMath {
                                                       const double* ivr, double rm2, double rm1, double rp1,
                                                       double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
                                                       Face<double>& fL, Face<double>& fR, int& nclL, int& nclR);
+    // the mixed form (R = float) samples the Riemann solution in FP32
+    template <class R>
     static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
                                                    int& nit)
     {
-        return fm::riemann_solve(Rs, L, Rt, nit);
+        return fm::riemann_solve<R>(Rs, L, Rt, nit);
     }
     static __device__ __forceinline__ void integrate(const IScal& S, const Flux& F1, const Flux& F2, double& p,
                                                      double& rho, double& u, double& v, double& w, double etot,
@@ -504,10 +507,17 @@ __device__ __forceinline__ void x_run(const BoxView& B, const AllScalars<R>& A, 
         M::template cell_faces<R, NV, GRAV, COSMO>(A, cd, iv, ivr, rm2, am1[VR], ap1[VR], rp2, dm1[VR], drp1,
                                                    am1[VP], ap1[VP], flat, fL, fR, nclL, nclR);
 
-        // Riemann problem at interface c-1/2: right face of c-1 | left face of c
+        // Riemann problem at interface c-1/2: right face of c-1 | left face of c;
+        // the updates of lanes 3..60 use the interfaces of lanes 3..61
         const Face<double> fRm1 = from_left(fR);
-        int nit;
-        Flux F = M::riemann(A.Rs, fRm1, fL, nit);
+        const bool solve = in_range && lane >= 3 && lane <= 61;
+        int nit = 0;
+        Flux F;
+        if (solve) {
+            F = M::template riemann<R>(A.Rs, fRm1, fL, nit);
+        } else {
+            F.p = F.r = F.u = F.v = F.w = 0.0;
+        }
         // high-density-peak floor (riemannsolver.F90:279-285), needs rho c-3..c
         const uint64_t M1 = __ballot(cd[VR] > A.Rs.dmax * am1[VR]);
         const uint64_t M2 = __ballot(am1[VR] > A.Rs.dmax * cd[VR]);
@@ -649,7 +659,7 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
             Flux F = Fprev;
             if (kk >= s0 + 2) {     // Riemann problem at interface k-2
                 int nit;
-                F = M::riemann(A.Rs, fRprev, fL, nit);
+                F = M::template riemann<R>(A.Rs, fRprev, fL, nit);
                 peak_floor(A.Rs, F, r5, w0[VR], w1[VR], w2[VR], w0[VU], w1[VU]);
                 // Newton count once per interface of the pencil (0 .. Nax), as
                 // the reference's loop over interfaces: a segment owns
