@@ -42,6 +42,7 @@ BYTES_PER_CELL_STEP = 3 * BYTES_PER_CELL_SWEEP
 NB = 3
 FIELDS = ("p", "rho", "vx", "vy", "vz")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu.json")
 
 
 def log(*a):
@@ -103,9 +104,24 @@ def roofline_of(r, N, prec):
         except (ValueError, KeyError):
             traffic = None
     per_dir = {names[d]: round(r["ms"][d] / max(r["launches"][d], 1), 4) for d in range(3)}
-    return {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "alg_bytes_per_launch": alg_bytes, "avg_ms_per_launch": per_dir}
+    out = {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "alg_bytes_per_launch": alg_bytes, "avg_ms_per_launch": per_dir}
+    # the issue floor: the dominant kernel's VALU instructions per launch (SQ
+    # counters of the same workload, profiles/valu.json) at one wave64
+    # instruction per 4 cycles on each of 1024 SIMDs at 2.4 GHz, against the
+    # launch time measured here
+    if os.path.exists(VALU_FILE):
+        try:
+            v = json.load(open(VALU_FILE)).get(f"{names[dom]}@{N}@{prec}")
+            if v:
+                out["valu_issue"] = {"insts_per_launch": v["valu_insts_per_launch"],
+                                     "insts_per_64_cells": v["valu_insts_per_64_cells"],
+                                     "issue_bound_ms": v["issue_bound_ms"],
+                                     "frac": round(v["issue_bound_ms"] / avg_ms, 4)}
+        except (ValueError, KeyError):
+            pass
+    return out
 
 
 def cpu_baseline(N, budget_s=10.0):

@@ -104,9 +104,15 @@ __global__ void __launch_bounds__(256) batch_x(const BatchBox* __restrict__ boxe
     if (cur >= 0) flush_counters(boxes[cur].st, t, slot == 2);
 }
 
+// At least two waves per SIMD for the gamma = 5/3 forms and the mixed forms:
+// left to itself the allocator gives the 5/3 FP64 z-sweep body 257 registers
+// and the general-gamma mixed bodies 259, just above the 256 that two waves
+// allow, and those kernels run at one wave per SIMD; held at two they fit
+// with no spill.  (The general-gamma FP64 bodies need ~290; forced to two
+// waves they spill about 1 KB per lane, so they are left to the allocator.)
 template <class M, class R, int DIR, bool EXP>
-__global__ void __launch_bounds__(256) batch_stream(const BatchBox* __restrict__ boxes,
-                                                    const BatchPlan* __restrict__ plan, int key, int slot)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((M::G53 || sizeof(R) == 4) ? 2 : 1)))
+batch_stream(const BatchBox* __restrict__ boxes, const BatchPlan* __restrict__ plan, int key, int slot)
 {
     const BatchTable& T = plan->t[key];
     const int total = T.total;

@@ -37,6 +37,7 @@ constexpr double K_0010 = (double)0.010f;        // dda.F90:200
 constexpr double K_0050 = (double)0.050f;        // dda.F90:203
 constexpr double K_1EM15 = (double)1e-15f;       // order2interp.F90:176-189
 constexpr double K_TERR = (double)1e-3f;         // riemann.F90:48
+constexpr int K_NNEWTMAX = 100;                   // riemann.F90:51
 
 template <class T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 template <class T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
@@ -267,6 +268,7 @@ struct RScal {
     double gamma, gamma1, rgamma, rgamma1, m, rm, gf, dadt, dmax;
     int m5;   // m == 1/5 (gamma = 5/3): powers of m, 1/gamma, 1/m are roots/products
     int nmax; // iterations after which the reference stops (nnewtmax = 100, riemann.F90:51)
+    int nskip; // K_NNEWTMAX - nmax: the fast iteration's head start for a limit other than 100
 };
 
 __device__ __forceinline__ double rarwave(double f1, double f2, double vt, double vh)

@@ -427,29 +427,28 @@ __device__ __forceinline__ void newton_side(const RScal& Rs, bool shock, double 
     }
 }
 
-// van Leer Newton iteration in FP64 (riemann.F90:58-109) from the iterate in
-// pstar, with nnewt iterations already done; the same iterates as the
-// reference up to rounding.  Runs until the relative change is within terr
-// (returns 1), until kmax iterations in all (returns 0: not finished; call
-// again to continue with the same iterate sequence), or past the reference's
-// limit (returns -1 with nnewt = -1, where riemann.F90:60-65 stops).  vstar
-// (returns 1 and -1) is the reference's (pl - pr + a vl + b vr)/(a + b) with
-// the last iteration's impedances, written with 1/a and 1/b.
-// cleft/cright are the Lagrangian sound speeds sqrt(gamma p rho) of the two
-// sides (riemann.F90:68-69), rcl/rcr their inverses.
-__device__ __forceinline__ int newton_resume(const RScal& Rs, double& pstar, double& vstar, int& nnewt, int kmax,
-                                             double pleft, double pright, double vxleft, double vxright,
-                                             double cleft, double rcl, double cright, double rcr)
+// van Leer Newton iteration in FP64 (riemann.F90:58-109) from the initial
+// iterate in pstar; the same iterates as the reference up to rounding.  Runs
+// until the relative change is within terr and returns the iterations done,
+// or -1 past the reference's limit (where riemann.F90:60-65 stops).  vstar
+// is the reference's (pl - pr + a vl + b vr)/(a + b) with the last
+// iteration's impedances, written with 1/a and 1/b.  cleft/cright are the
+// Lagrangian sound speeds sqrt(gamma p rho) of the two sides
+// (riemann.F90:68-69), rcl/rcr their inverses.
+// The loop's limit is the reference's constant nnewtmax = 100; a different
+// limit set through the API enters as a head start of the counter
+// (Rs.nskip = 100 - limit), which keeps a runtime bound out of the loop.
+__device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vstar, double pleft, double pright,
+                                      double vxleft, double vxright, double cleft, double rcl, double cright,
+                                      double rcr)
 {
     const double cl2 = cleft * cleft, cr2 = cright * cright;
     const double rpl = rcp(pleft), rpr = rcp(pright);
     double ra = 0.0, rb = 0.0;
-    int st = 1;
+    int k = Rs.nskip;
     for (;;) {
-        if (nnewt >= kmax) return 0;
-        if (++nnewt > Rs.nmax) {                     // where the reference stops (riemann.F90:60-65)
-            nnewt = -1;
-            st = -1;
+        if (++k > K_NNEWTMAX) {                      // where the reference stops (riemann.F90:60-65)
+            k = -1;
             break;
         }
         const double pstarold = pstar;
@@ -464,17 +463,7 @@ __device__ __forceinline__ int newton_resume(const RScal& Rs, double& pstar, dou
         if (!(fabs(pstarold - pstar) > K_TERR * pstarold)) break;
     }
     vstar = div(fma((pleft - pright) * ra, rb, fma(vxleft, rb, vxright * ra)), ra + rb);
-    return st;
-}
-
-// The whole iteration from the initial guess: iterations done, or -1.
-__device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vstar, double pleft, double pright,
-                                      double vxleft, double vxright, double cleft, double rcl, double cright,
-                                      double rcr)
-{
-    int nnewt = 0;
-    newton_resume(Rs, pstar, vstar, nnewt, 0x7fffffff, pleft, pright, vxleft, vxright, cleft, rcl, cright, rcr);
-    return nnewt;
+    return k < 0 ? -1 : k - Rs.nskip;
 }
 
 template <class R> __device__ __forceinline__ R rarwave(R f1, R f2, R vt, R vh)
@@ -548,12 +537,13 @@ __device__ __forceinline__ void riemann_setup(const RScal& Rs, const Face<double
     }
 }
 
-// Newton iterations of patterns 1 and 2 from S.pm (newton_resume's contract).
-__device__ __forceinline__ int riemann_newton(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
-                                              RSolve& S, int kmax)
+// Newton iterations of patterns 1 and 2 from S.pm: S.pm, S.vxm the star
+// state, S.nit the iterations (-1 where the reference stops).
+__device__ __forceinline__ void riemann_newton(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
+                                               RSolve& S)
 {
-    return newton_resume(Rs, S.pm, S.vxm, S.nit, kmax, L.p, Rt.p, L.u, Rt.u, L.r * S.cleft, S.rcle * S.rrl,
-                         Rt.r * S.cright, S.rcre * S.rrr);
+    S.nit = newton(Rs, S.pm, S.vxm, L.p, Rt.p, L.u, Rt.u, L.r * S.cleft, S.rcle * S.rrl, Rt.r * S.cright,
+                   S.rcre * S.rrr);
 }
 
 // The costly sampling operations in precision Q.  Q = double: the FP64
@@ -752,7 +742,7 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
         S.nit = 1;
     } else {
         riemann_setup(Rs, L, Rt, S);
-        if (S.kase == 1 || S.kase == 2) riemann_newton(Rs, L, Rt, S, 0x7fffffff);
+        if (S.kase == 1 || S.kase == 2) riemann_newton(Rs, L, Rt, S);
     }
     nit = S.nit;
     return riemann_sample<Q>(Rs, L, Rt, S);

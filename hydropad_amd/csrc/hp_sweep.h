@@ -206,7 +206,8 @@ __device__ __forceinline__ AllScalars<R> load_scalars(const Params* __restrict__
     A.Rs.dadt = 1.0 - 0.5 * P.dat * P.dt * P.rat;     // riemannsolver.F90:36
     A.Rs.dmax = P.dmax;
     A.Rs.m5 = m5;
-    A.Rs.nmax = cap > 0 ? cap : 100;
+    A.Rs.nmax = cap > 0 ? cap : K_NNEWTMAX;
+    A.Rs.nskip = K_NNEWTMAX - A.Rs.nmax;
     A.Is.rdtath = P.rdtath; A.Is.rdx = P.rdx; A.Is.gamma = gamma; A.Is.rgamma1 = rgamma1;
     A.Is.eta1 = P.eta1; A.Is.eta2 = P.eta2;
     A.rgamma1 = rgamma1; A.gamma = gamma; A.mingradflat = P.mingradflat;
