@@ -310,13 +310,36 @@ def run_serving(nreq=48, rate=8.0, seed=5150):
                      else "fp64", "seed": int(rng.integers(1 << 62)), "arrival_s": t})
     res, stats, _ = batch.serve(reqs, hbm_budget=64 << 30, max_boxes=64, use_graph=True)
     lat = np.array([r["done_s"] - r["arrival_s"] for r in res if r["rc"] == 0])
-    return {"workload": f"poisson_{rate}req/s_{nreq}req_N32-256_steps20-200_sod_seeded",
-            "requests": nreq, "rate_per_s": rate, "completed": int(stats["completed"]),
-            "cell_updates_per_s": round(stats["cell_updates"] / stats["wall_s"], 1),
-            "wall_s": round(stats["wall_s"], 3), "p50_latency_s": round(stats["p50_s"], 4),
-            "p99_latency_s": round(stats["p99_s"], 4), "mean_latency_s": round(float(lat.mean()), 4),
-            "batched_steps": int(stats["batched_steps"]), "max_concurrent": int(stats["max_concurrent"]),
-            "peak_hbm_gb": round(stats["peak_bytes"] / 1e9, 3)}
+    out = {"workload": f"poisson_{rate}req/s_{nreq}req_N32-256_steps20-200_sod_seeded",
+           "requests": nreq, "rate_per_s": rate, "completed": int(stats["completed"]),
+           "cell_updates_per_s": round(stats["cell_updates"] / stats["wall_s"], 1),
+           "wall_s": round(stats["wall_s"], 3), "p50_latency_s": round(stats["p50_s"], 4),
+           "p99_latency_s": round(stats["p99_s"], 4), "mean_latency_s": round(float(lat.mean()), 4),
+           "batched_steps": int(stats["batched_steps"]), "max_concurrent": int(stats["max_concurrent"]),
+           "peak_hbm_gb": round(stats["peak_bytes"] / 1e9, 3)}
+    # The serving loop's own cost, apart from the arrival process: the same
+    # requests all present at t = 0 through the serving loop (admission,
+    # one-step-ahead queueing, completion) against one plain batch holding
+    # them from the start and stepped until every box is done
+    sat, st2, _ = batch.serve([dict(r, arrival_s=0.0) for r in reqs], hbm_budget=64 << 30, max_boxes=64,
+                              use_graph=True)
+    assert all(r["rc"] == 0 for r in sat)
+    cells = sum(r["n"] ** 3 * r["nsteps"] for r in reqs)
+    with batch.Batch(max_boxes=64, hbm_budget=64 << 30) as b:
+        for r in reqs:
+            b.add_sod(r["n"], r["seed"], r["nsteps"], r["precision"])
+        b.use_graph(True)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run_until_done(chunk=8)
+        b.sync()
+        offline_s = time.perf_counter() - t0
+    out["saturated"] = {"cell_updates_per_s": round(st2["cell_updates"] / st2["wall_s"], 1),
+                        "wall_s": round(st2["wall_s"], 3),
+                        "offline_batch_cell_updates_per_s": round(cells / offline_s, 1),
+                        "offline_batch_wall_s": round(offline_s, 3),
+                        "serving_vs_offline_batch": round(st2["cell_updates"] / st2["wall_s"] / (cells / offline_s), 4)}
+    return out
 
 
 def main():
