@@ -66,7 +66,7 @@ class HydropadError(RuntimeError):
 
 # every symbol include/hydropad_gpu.h declares (checked by tests/test_capi_symbols.py)
 EXPORTS = ("hp_params_init", "hp_create", "hp_destroy", "hp_set_precision", "hp_upload",
-           "hp_download", "hp_upload_device", "hp_download_device", "hp_set_params",
+           "hp_download", "hp_upload_device", "hp_upload_gforce", "hp_download_device", "hp_set_params",
            "hp_ppmsolver", "hp_exchange_mesh", "hp_ppm_lxlylz", "hp_speed", "hp_evolve",
            "hp_status", "hp_reset_counters", "hp_set_newton_limit", "hp_sync", "hp_integrate_cell", "hp_dualenergy_cell",
            "hp_sweep_timing_enable", "hp_sweep_timing", "hp_set_tiling", "hp_cosmo_init", "hp_timedet_cosmo",
@@ -97,6 +97,7 @@ def lib():
     L.hp_upload.argtypes = [_P] + [_P] * 6
     L.hp_download.argtypes = [_P] + [_P] * 5
     L.hp_upload_device.argtypes = [_P] + [_P] * 6
+    L.hp_upload_gforce.argtypes = [_P, _P]
     L.hp_download_device.argtypes = [_P] + [_P] * 5
     L.hp_set_params.argtypes = [_P, PP]
     L.hp_ppmsolver.argtypes = [_P, _I]

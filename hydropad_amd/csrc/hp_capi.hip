@@ -324,6 +324,16 @@ int hp_upload(hp_ctx* c, const double* p, const double* rho, const double* vx, c
     return upload_impl(c, s, gforce, hipMemcpyHostToDevice);
 }
 
+int hp_upload_gforce(hp_ctx* c, const double* gforce)
+{
+    if (!c) return HP_E_ARG;
+    if (gforce)
+        HPCHK(hipMemcpyAsync(c->gforce, gforce, 3 * c->ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->has_grav = gforce != nullptr;
+    HPCHK(hipStreamSynchronize(c->stream));
+    return HP_OK;
+}
+
 int hp_upload_device(hp_ctx* c, const double* p, const double* rho, const double* vx, const double* vy,
                      const double* vz, const double* gforce)
 {

@@ -6,13 +6,21 @@
 ! (hydropad/main.F90:125-136 + io/write_var_mpi.F90: rho.bin, pressure.bin,
 ! vx.bin, vy.bin, vz.bin; REAL(4) interior, Fortran order, no header).
 !
-!   hydropad_driver N NSTEPS [fp64|mixed|exact] [OUTDIR] [resident|sweeps] [NEWTON_LIMIT]
+!   hydropad_driver N NSTEPS [fp64|mixed|exact] [OUTDIR] [resident|sweeps|cells|nbody] [NEWTON_LIMIT] [...]
 !
 ! NEWTON_LIMIT (default 0 = the reference's 100) is the Riemann Newton
 ! iteration count past which the run stops as hydro/riemann.F90:60-65 does.
 ! Mode "cells" runs no time loop: it calls the integrate_gpu drop-in on one
 ! cold cell with equal face fluxes for the internal energies given as the
 ! remaining arguments and prints each new pressure.
+! Mode "nbody" runs the N-body drop-ins over NSTEPS steps of kick, drift,
+! kick (hydropad/evolve_sys.F90:63-84) with dtat = 0.015, dat = 0.2,
+! rdtath = 0.04, on gforce.bin (3,nx,ny,nz) and NP particles ppos.bin,
+! pvel.bin (3,NP) REAL(8) read from OUTDIR (NP is argument 7): once with the
+! per-call wrappers kick_gpu / drift_gpu (host arrays in and out, as the
+! reference calls kick and drift) into ppos_call.bin / pvel_call.bin, once
+! device-resident (particles attached, nbody_step_gpu per step) into
+! ppos_res.bin / pvel_res.bin, and prints the particles-outside counter.
 ! "resident" keeps the state on the GPU for the whole run (attach /
 ! ppm_lxlylz_gpu / speed_gpu / detach); "sweeps" calls the per-sweep drop-in
 ! ppmsolver_gpu with host arrays exactly where hydro/ppm_lxlylz.F90 calls
@@ -84,6 +92,11 @@ program hydropad_driver
     call hydropad_gpu_finalize()
     stop
   end if
+  if (trim(mode) == 'nbody') then
+    call nbody_run()
+    call hydropad_gpu_finalize()
+    stop
+  end if
   if (trim(mode) == 'sweeps') then
     allocate(gforce(3, nx, ny, nz))
     gforce = 0.0d0
@@ -137,6 +150,58 @@ contains
       write(*, '(a,es26.17e3,a,es26.17e3)') 'integrate: eint=', eint, ' pres=', pres
     end do
   end subroutine integrate_cells
+
+  ! kick, drift, kick per step through the per-call and the resident drop-ins
+  subroutine nbody_run()
+    integer :: np, s
+    real(c_double), allocatable :: g(:,:,:,:), x0(:,:), v0(:,:), x(:,:), v(:,:)
+    type(hp_counters_t) :: cnt
+    call get_command_argument(7, arg); read(arg, *) np
+    allocate(g(3, nx, ny, nz), x0(3, np), v0(3, np), x(3, np), v(3, np))
+    call read_raw(g, size(g), 'gforce.bin')
+    call read_raw(x0, size(x0), 'ppos.bin')
+    call read_raw(v0, size(v0), 'pvel.bin')
+    call hydropad_gpu_set_scalars(mingradflat, flatvalue, gamma, 1.0d0, m, rgamma1, rm, rgamma, dmax, &
+                                  rdx, 0.04d0, 0.0d0, dx, 0.2d0, 1.0d0, gamma1, gf, eta1, eta2)
+    call hydropad_gpu_set_dtat(0.015d0)
+    x = x0; v = v0
+    do s = 1, nsteps                      ! hydropad/evolve_sys.F90:63-84, one rank
+      call kick_gpu(np, x, v, nx, ny, nz, g)
+      call drift_gpu(np, x, v)
+      call kick_gpu(np, x, v, nx, ny, nz, g)
+    end do
+    call write_raw(x, size(x), 'ppos_call.bin'); call write_raw(v, size(v), 'pvel_call.bin')
+    call hydropad_gpu_attach(p3d, rho3d, vx3d, vy3d, vz3d, nx, ny, nz, nb)
+    call hydropad_gpu_set_gforce(g)
+    call hydropad_gpu_particles_attach(np, x0, v0)
+    do s = 1, nsteps
+      call nbody_step_gpu()
+    end do
+    call hydropad_gpu_particles_detach(x, v)
+    call write_raw(x, size(x), 'ppos_res.bin'); call write_raw(v, size(v), 'pvel_res.bin')
+    call hydropad_gpu_counters(cnt)
+    write(*, '(a,i0)') 'hydropad_driver: particles_outside=', cnt%particles_outside
+  end subroutine nbody_run
+
+  subroutine read_raw(a, n, name)
+    integer, intent(in) :: n
+    real(c_double), intent(out) :: a(n)
+    character(len=*), intent(in) :: name
+    integer :: u
+    open(newunit=u, file=trim(outdir)//'/'//name, access='stream', form='unformatted', status='old')
+    read(u) a
+    close(u)
+  end subroutine read_raw
+
+  subroutine write_raw(a, n, name)
+    integer, intent(in) :: n
+    real(c_double), intent(in) :: a(n)
+    character(len=*), intent(in) :: name
+    integer :: u
+    open(newunit=u, file=trim(outdir)//'/'//name, access='stream', form='unformatted', status='replace')
+    write(u) a
+    close(u)
+  end subroutine write_raw
 
   ! CFL maximum on the host, hydropad/speed.F90:22-31
   real(c_double) function host_speed()

@@ -13,10 +13,21 @@
 !                 dmax,rdx,rdtath,dt,dx,dat,at,gamma1,gf,eta1,eta2)
 !       = the scalar push before every sweep (hydro/ppmsolver.F90:68-69), same order
 !
+!   kick_gpu(np,ppos,pvel,ngx,ngy,ngz,gforce)
+!       = SUBROUTINE kick (nbody/kick.F90:1)
+!   drift_gpu(np,ppos,pvel)
+!       = SUBROUTINE drift (nbody/drift.F90:1)
+!   hydropad_gpu_set_dtat(dtat)
+!       = the module scalar dtat the kicks read (hydropad/scalar.F90:18)
+!
 ! and a device-resident form for an evolve_sys-shaped time loop:
 !   hydropad_gpu_attach(p3d,rho3d,vx3d,vy3d,vz3d,nx,ny,nz,nbound)   upload once
+!   hydropad_gpu_set_gforce(gforce)     gravity array, once (initialize_gravity)
+!   hydropad_gpu_particles_attach(np,ppos,pvel)                     upload once
 !   ppm_lxlylz_gpu(nrot)                (hydro/ppm_lxlylz.F90, 3 sweeps + ghost fills)
-!   speed_gpu(velmax)                   (hydropad/speed.F90)
+!   nbody_step_gpu()                    (kick, drift, kick: evolve_sys.F90:63-84)
+!   speed_gpu(velmax)                   (hydropad/speed.F90, particles included)
+!   hydropad_gpu_particles_detach(ppos,pvel)                        download once
 !   hydropad_gpu_detach(p3d,rho3d,vx3d,vy3d,vz3d)                  download once
 module hydropad_gpu
   use iso_c_binding
@@ -43,7 +54,10 @@ module hydropad_gpu
   public :: hydropad_gpu_set_scalars, hydropad_gpu_set_precision, hydropad_gpu_set_newton_limit, &
             ppmsolver_gpu, integrate_gpu, &
             dualenergy_gpu, hydropad_gpu_attach, ppm_lxlylz_gpu, speed_gpu, exchange_mesh_gpu, &
-            hydropad_gpu_detach, hydropad_gpu_finalize, hydropad_gpu_check
+            hydropad_gpu_detach, hydropad_gpu_finalize, hydropad_gpu_check, &
+            hydropad_gpu_set_dtat, kick_gpu, drift_gpu, hydropad_gpu_set_gforce, &
+            hydropad_gpu_particles_attach, nbody_step_gpu, hydropad_gpu_particles_detach, &
+            hydropad_gpu_counters
 
   interface
     integer(c_int) function hp_params_init(gamma, ngrid_max, dt, out) bind(C, name="hp_params_init")
@@ -109,6 +123,31 @@ module hydropad_gpu
       real(c_double), value :: cour, tstop
       real(c_double), intent(inout) :: t, dt
     end function
+    integer(c_int) function hp_upload_gforce(ctx, gforce) bind(C, name="hp_upload_gforce")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, gforce
+    end function
+    integer(c_int) function hp_particles_upload(ctx, np, ppos, pvel, dtat) bind(C, name="hp_particles_upload")
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: np
+      real(c_double), intent(in) :: ppos(*), pvel(*)
+      real(c_double), value :: dtat
+    end function
+    integer(c_int) function hp_particles_download(ctx, ppos, pvel) bind(C, name="hp_particles_download")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ppos(*), pvel(*)
+    end function
+    integer(c_int) function hp_kick(ctx, dtat) bind(C, name="hp_kick")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dtat
+    end function
+    integer(c_int) function hp_drift(ctx) bind(C, name="hp_drift")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
     integer(c_int) function hp_set_newton_limit(ctx, nmax) bind(C, name="hp_set_newton_limit")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -143,6 +182,7 @@ module hydropad_gpu
   integer(c_int), save :: cprec = HP_FP64
   type(hp_params_t), save :: prm
   integer(c_int), save :: cnewt = 0
+  real(c_double), save :: cdtat = 0.0d0
 
 contains
 
@@ -275,6 +315,86 @@ contains
     real(c_double), intent(out) :: p3d(*), rho3d(*), vx3d(*), vy3d(*), vz3d(*)
     call hydropad_gpu_check(hp_download(ctx, p3d, rho3d, vx3d, vy3d, vz3d), 'hp_download')
   end subroutine hydropad_gpu_detach
+
+  ! The kicks' time step, the reference's module scalar dtat
+  ! (hydropad/scalar.F90:18, set at initialize_parameters.F90:174,193).
+  subroutine hydropad_gpu_set_dtat(dtat)
+    real(c_double), intent(in) :: dtat
+    cdtat = dtat
+  end subroutine hydropad_gpu_set_dtat
+
+  ! Drop-in for SUBROUTINE kick (nbody/kick.F90:1, one rank): host particles and
+  ! gravity array in, v <- v (1 - dat dtat/2) + F dtat/2 on the GPU, velocities
+  ! out.  ngx, ngy, ngz are the array dimensions (ghosts included); the grid's
+  ! nbound is the attached context's, else the reference's 3.
+  subroutine kick_gpu(np, ppos, pvel, ngx, ngy, ngz, gforce)
+    integer, intent(in) :: np, ngx, ngy, ngz
+    real(c_double), intent(in) :: ppos(3, np)
+    real(c_double), intent(inout) :: pvel(3, np)
+    real(c_double), intent(in), target :: gforce(3, ngx, ngy, ngz)
+    real(c_double), allocatable :: pos(:, :)
+    integer :: nb
+    nb = 3
+    if (c_associated(ctx) .and. ngx == cnx .and. ngy == cny .and. ngz == cnz) nb = cnb
+    call ensure_ctx(ngx, ngy, ngz, nb)
+    call hydropad_gpu_check(hp_upload_gforce(ctx, c_loc(gforce)), 'hp_upload_gforce')
+    call hydropad_gpu_check(hp_particles_upload(ctx, int(np, c_int64_t), ppos, pvel, cdtat), 'hp_particles_upload')
+    call hydropad_gpu_check(hp_kick(ctx, cdtat), 'hp_kick')
+    allocate(pos(3, np))
+    call hydropad_gpu_check(hp_particles_download(ctx, pos, pvel), 'hp_particles_download')
+  end subroutine kick_gpu
+
+  ! Drop-in for SUBROUTINE drift (nbody/drift.F90:1, one rank): x <- x + rdtath v
+  ! wrapped once into the box, on the grid of the context a hydro call or
+  ! kick_gpu set up (the box is [0, ngrid/max(ngrid)) per axis).
+  subroutine drift_gpu(np, ppos, pvel)
+    integer, intent(in) :: np
+    real(c_double), intent(inout) :: ppos(3, np)
+    real(c_double), intent(in) :: pvel(3, np)
+    real(c_double), allocatable :: vel(:, :)
+    if (.not. c_associated(ctx)) then
+      write(*,*) 'hydropad_gpu: drift_gpu needs the grid of a context (a hydro call or kick_gpu first)'
+      stop 1
+    end if
+    call hydropad_gpu_check(hp_set_params(ctx, prm), 'hp_set_params')
+    call hydropad_gpu_check(hp_particles_upload(ctx, int(np, c_int64_t), ppos, pvel, cdtat), 'hp_particles_upload')
+    call hydropad_gpu_check(hp_drift(ctx), 'hp_drift')
+    allocate(vel(3, np))
+    call hydropad_gpu_check(hp_particles_download(ctx, ppos, vel), 'hp_particles_download')
+  end subroutine drift_gpu
+
+  ! Device-resident N-body: the gravity array once ...
+  subroutine hydropad_gpu_set_gforce(gforce)
+    real(c_double), intent(in), target :: gforce(3, cnx, cny, cnz)
+    call hydropad_gpu_check(hp_upload_gforce(ctx, c_loc(gforce)), 'hp_upload_gforce')
+  end subroutine hydropad_gpu_set_gforce
+
+  ! ... the particles once (the attached context's grid) ...
+  subroutine hydropad_gpu_particles_attach(np, ppos, pvel)
+    integer, intent(in) :: np
+    real(c_double), intent(in) :: ppos(3, np), pvel(3, np)
+    call hydropad_gpu_check(hp_particles_upload(ctx, int(np, c_int64_t), ppos, pvel, cdtat), 'hp_particles_upload')
+  end subroutine hydropad_gpu_particles_attach
+
+  ! ... kick, drift, kick of one step (hydropad/evolve_sys.F90:63-84) ...
+  subroutine nbody_step_gpu()
+    call hydropad_gpu_check(hp_set_params(ctx, prm), 'hp_set_params')
+    call hydropad_gpu_check(hp_kick(ctx, cdtat), 'hp_kick')
+    call hydropad_gpu_check(hp_drift(ctx), 'hp_drift')
+    call hydropad_gpu_check(hp_kick(ctx, cdtat), 'hp_kick')
+  end subroutine nbody_step_gpu
+
+  ! ... and the particles back at the end.
+  subroutine hydropad_gpu_particles_detach(ppos, pvel)
+    real(c_double), intent(out) :: ppos(*), pvel(*)
+    call hydropad_gpu_check(hp_particles_download(ctx, ppos, pvel), 'hp_particles_download')
+  end subroutine hydropad_gpu_particles_detach
+
+  ! The context's counters (Newton, clamps, particles outside the array).
+  subroutine hydropad_gpu_counters(cnt)
+    type(hp_counters_t), intent(out) :: cnt
+    call hydropad_gpu_check(hp_status(ctx, cnt), 'hp_status')
+  end subroutine hydropad_gpu_counters
 
   subroutine hydropad_gpu_finalize()
     if (c_associated(ctx)) call hp_destroy(ctx)

@@ -107,6 +107,16 @@ class Box:
         check("hp_upload", self.L.hp_upload(self._h, *[_ptr(a) for a in arrs],
                                              _ptr(g) if g is not None else None))
 
+    def upload_gforce(self, gforce):
+        """The gravity array alone ((3, nx, ny, nz) or None for none); the
+        fields stay as they are."""
+        g = None
+        if gforce is not None:
+            g = np.asfortranarray(gforce, dtype=np.float64)
+            if g.shape != (3,) + self.shape:
+                raise ValueError("gforce must be (3, nx, ny, nz)")
+        check("hp_upload_gforce", self.L.hp_upload_gforce(self._h, _ptr(g) if g is not None else None))
+
     def download(self):
         out = {k: np.empty(self.shape, dtype=np.float64, order="F") for k in FIELDS}
         check("hp_download", self.L.hp_download(self._h, *[_ptr(out[k]) for k in FIELDS]))

@@ -90,6 +90,12 @@ int hp_upload(hp_ctx* ctx, const double* p, const double* rho, const double* vx,
               const double* vy, const double* vz, const double* gforce);
 int hp_download(hp_ctx* ctx, double* p, double* rho, double* vx, double* vy, double* vz);
 
+/* The gravity array alone, host (3,nx,ny,nz) REAL(8), or NULL for none; the
+ * fields are left as they are.  The reference sets gforce once
+ * (hydropad/initialize_gravity.F90:25-58) and both the sweeps
+ * (hydro/ppmsolver.F90:108,151) and kick (nbody/kick.F90:1) read it. */
+int hp_upload_gforce(hp_ctx* ctx, const double* gforce);
+
 /* Device-to-device variants: pointers are device (or managed) memory. */
 int hp_upload_device(hp_ctx* ctx, const double* p, const double* rho, const double* vx,
                      const double* vy, const double* vz, const double* gforce);
