@@ -680,23 +680,34 @@ int hp_batch_autotune(hp_batch* b, int steps_per_trial, hp_batch_tuning* best, d
         run += steps_per_trial;
         return HP_OK;
     };
+    // Each trial advances the boxes, and a step's cost drifts as the fields
+    // evolve (more Newton iterations as structure forms), so the candidates of
+    // a parameter are timed in order and again in reverse order and each is
+    // scored by the mean of its two times: a drift linear in the step count
+    // adds the same to every candidate's mean.
     int rc = HP_OK;
     for (int param = 0; param < 3 && rc == HP_OK; ++param) {
         const int* cand = param == 0 ? XC : (param == 1 ? YS : WP);
         const int nc = param == 0 ? 5 : 3;
-        hp_batch_tuning pick = cur;
-        double pick_ms = 1e300;
-        for (int c = 0; c < nc && rc == HP_OK; ++c) {
+        double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < 2 * nc && rc == HP_OK; ++k) {
+            const int c = k < nc ? k : 2 * nc - 1 - k;
             hp_batch_tuning t = cur;
             if (param == 0) t.x_chunks = cand[c];
             else if (param == 1) t.yz_segments = cand[c];
             else t.waves_per_cu = cand[c];
             double ms = 0.0;
             rc = trial(t, ms);
-            if (rc == HP_OK && ms < pick_ms) { pick_ms = ms; pick = t; }
+            sum[c] += ms;
         }
-        cur = pick;
-        best_ms = pick_ms;
+        if (rc) break;
+        int pick = 0;
+        for (int c = 1; c < nc; ++c)
+            if (sum[c] < sum[pick]) pick = c;
+        if (param == 0) cur.x_chunks = cand[pick];
+        else if (param == 1) cur.yz_segments = cand[pick];
+        else cur.waves_per_cu = cand[pick];
+        best_ms = 0.5 * sum[pick];
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);

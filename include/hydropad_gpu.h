@@ -306,8 +306,10 @@ int hp_batch_get_tuning(hp_batch* b, hp_batch_tuning* t);
 size_t hp_batch_bytes_used(const hp_batch* b);
 
 /* Autotuner: coordinate search over x_chunks, yz_segments and waves_per_cu,
- * each trial steps_per_trial real batched steps timed with HIP events; leaves
- * the fastest tiling set.  *steps_run receives the steps it advanced. */
+ * each trial steps_per_trial real batched steps timed with HIP events, the
+ * candidates of each parameter timed in order and in reverse order and
+ * scored by their mean (22 trials in all); leaves the fastest tiling set.
+ * *steps_run receives the steps it advanced. */
 int hp_batch_autotune(hp_batch* b, int steps_per_trial, hp_batch_tuning* best,
                       double* best_ms_per_step, int* steps_run);
 

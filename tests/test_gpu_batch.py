@@ -107,7 +107,7 @@ def test_autotune_keeps_results():
         ia = [a.add(copy_state(s), nsteps=40) for s in sts]
         it = [t.add(copy_state(s), nsteps=40) for s in sts]
         best, ms, n = t.autotune(steps_per_trial=1)
-        assert n == 11 and ms > 0.0
+        assert n == 22 and ms > 0.0
         assert best["x_chunks"] in (1, 2, 4, 8, 16) and best["waves_per_cu"] in (4, 8, 16)
         a.step(n)
         for x, y in zip(ia, it):
