@@ -264,3 +264,35 @@ def test_newton_stop_in_the_fortran_driver(tmp_path):
                        timeout=120)
     assert "convergenza non raggiunta" in r.stdout, r.stdout + r.stderr
     assert not (tmp_path / "rho.bin").exists()
+
+
+def test_create_defaults_use_the_longest_axis():
+    """hp_create seeds the scalars as initialize_parameters.F90:23 does, with
+    ngrid = max(ngridx, ngridy, ngridz): a non-cubic box stepped through the
+    bare C-ABI with the defaults matches the same box whose scalars were set
+    from params_init(max extent), bit for bit."""
+    import ctypes
+    ngrid = (12, 16, 10)
+    shape = tuple(n + 2 * NB for n in ngrid)
+    rng = np.random.default_rng(12)
+    st = {k: np.asfortranarray(rng.uniform(0.5, 1.5, shape) if k in ("p", "rho") else
+                               rng.uniform(-0.2, 0.2, shape)) for k in FIELDS}
+    with hp.Box(shape) as box:
+        box.upload(copy_state(st))
+        dt0, t0 = box.timedet(box.speed(), 0.8)
+        t, dt = box.evolve(1, 3, 0.8, t0, dt0)
+        want = box.download()
+    L = _capi.lib()
+    h = ctypes.c_void_p()
+    _capi.check("hp_create", L.hp_create(*shape, NB, 0, ctypes.byref(h)))
+    try:
+        a = [np.asfortranarray(st[k].copy()) for k in FIELDS]
+        _capi.check("hp_upload", L.hp_upload(h, *[x.ctypes.data for x in a], None))
+        tt, dd = ctypes.c_double(t0), ctypes.c_double(dt0)
+        _capi.check("hp_evolve", L.hp_evolve(h, 1, 3, 0.8, 1e30, ctypes.byref(tt), ctypes.byref(dd)))
+        _capi.check("hp_download", L.hp_download(h, *[x.ctypes.data for x in a]))
+    finally:
+        L.hp_destroy(h)
+    assert (tt.value, dd.value) == (t, dt)
+    for k, x in zip(FIELDS, a):
+        assert np.array_equal(x, want[k]), k
