@@ -10,7 +10,7 @@ seconds) with the state resident in HBM, and the mixed form's per-field
 relative L1/Linf against the FP64 run.
 
     python bench.py [--gpus 1] [--steps 300] [--warmup 5] [--size 512] [--no-extras]
-                    [--legs cosmo,ensemble,serving]
+                    [--legs cosmo,ensemble,serving,slab]
 
 Prints one JSON line.  `roofline` is the dominant sweep kernel's achieved HBM
 bandwidth (80 algorithmic bytes per cell per sweep / its average HIP-event
@@ -24,6 +24,8 @@ workloads of the metric:
               P(k) slopes and rms, one batch, autotuned, hipGraph; FP64 and mixed
   serving     Poisson stream of Sod-tube requests (N log-uniform 32..256,
               20..200 steps, FP64 or mixed), continuous batching; p50/p99 latency
+  zslab_share one GPU's z-slab of the 512^3 tube split over 8 GPUs (512x512x64,
+              ghosts filled locally), FP64 and mixed
 """
 import argparse
 import json
@@ -302,6 +304,35 @@ def run_ensemble(hp, ic, steps=50, nbox=64):
     return out
 
 
+def run_slab(hp, N=512, nz=64, steps=300, cour=0.8):
+    """One GPU's share of the 512^3 tube split into z-slabs across 8 GPUs: a
+    512 x 512 x 64 box (dx = 1/512, the same planar shock tube in x) stepped
+    with its z ghosts filled locally, the work a rank would do between halo
+    exchanges.  The exchange itself (3 planes x 5 fields each side before a
+    z sweep, 2 x 3 x 5 x 8 x 512^2 B = 63 MB) is not run: one GPU here."""
+    shape = (N + 2 * NB, N + 2 * NB, nz + 2 * NB)
+    xc = (np.arange(shape[0]) - NB + 0.5) / N
+    left = (xc < 0.5)[:, None, None]
+    st = {"p": np.where(left, 1.0, 0.1), "rho": np.where(left, 1.0, 0.125)}
+    for k in ("vx", "vy", "vz"):
+        st[k] = np.zeros(shape)
+    st = {k: np.asfortranarray(np.broadcast_to(v, shape), dtype=np.float64) for k, v in st.items()}
+    out = {"workload": f"zslab_{N}x{N}x{nz}_sod_{steps}steps", "shape": [N, N, nz], "steps": steps,
+           "halo_bytes_per_z_sweep": 2 * NB * 5 * 8 * N * N}
+    for prec in ("fp64", "mixed"):
+        with hp.Box(shape, prec) as box:
+            box.upload(st)
+            dt, t = box.timedet(box.speed(), cour)
+            box.sync()
+            t0 = time.perf_counter()
+            box.evolve(1, steps, cour, t, dt)
+            box.sync()
+            wall = time.perf_counter() - t0
+        out[f"{prec}_cell_updates_per_s"] = round(N * N * nz * steps / wall, 1)
+        out[f"{prec}_ms_per_step"] = round(wall / steps * 1e3, 4)
+    return out
+
+
 def run_serving(nreq=48, rate=8.0, seed=5150):
     """Poisson stream of box evolutions through the continuous-batching loop."""
     from hydropad_amd import batch
@@ -357,12 +388,13 @@ def main():
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
-    ap.add_argument("--legs", default="cosmo,ensemble,serving",
+    ap.add_argument("--legs", default="cosmo,ensemble,serving,slab",
                     help="comma-separated other workloads to run beside the headline")
     args = ap.parse_args()
     legs = set() if args.no_extras else {x.strip() for x in args.legs.split(",") if x.strip()}
-    if legs - {"cosmo", "ensemble", "serving"}:
-        ap.error(f"unknown legs: {sorted(legs - {'cosmo', 'ensemble', 'serving'})}")
+    known = {"cosmo", "ensemble", "serving", "slab"}
+    if legs - known:
+        ap.error(f"unknown legs: {sorted(legs - known)}")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note, not a code path)"}))
@@ -421,6 +453,9 @@ def main():
     if "serving" in legs:
         log("bench: serving")
         line["serving"] = run_serving()
+    if "slab" in legs:
+        log("bench: z-slab share")
+        line["zslab_share"] = run_slab(hp, N=N, steps=K, cour=args.cour)
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline(N)
