@@ -548,14 +548,16 @@ __device__ __forceinline__ void riemann_newton(const RScal& Rs, const Face<doubl
 
 // The costly sampling operations in precision Q.  Q = double: the FP64
 // form's (the reference's algebra, divisions and roots to within an ulp).
-// Q = float: the mixed form, which takes the shock density, the star-region
-// sound speed and the fan interpolation weight in FP32 (v_rcp_f32,
-// v_sqrt_f32).  Each of those FP32 results scales a difference (the density
-// jump across the shock, the distance of a wave speed from 0, f1 - f2) or
-// only steers a switch, so its rounding stays at FP32 relative size of that
+// Q = float: the mixed form's, at single-precision accuracy -- the raw
+// v_rcp_f64 / v_rsq_f64 results (about 2^-24 relative) without the Newton
+// refinement, in FP64 registers, so no value leaves FP64's range -- for the
+// shock density, the star-region sound speed and the fan interpolation
+// weight.  Each of those scales a difference (the density jump across the
+// shock, the distance of a wave speed from 0, f1 - f2) or only steers a
+// switch, so its rounding stays at single-precision relative size of that
 // difference.  The rarefaction density rho (pm/p)^(1/gamma) and the
-// two-rarefaction star state stay in FP64: FP32 powers there carry FP32
-// rounding of the full state, and over 300 steps of the 512^3 Sod tube that
+// two-rarefaction star state stay at FP64 accuracy: single-precision powers
+// there round the full state, and over 300 steps of the 512^3 Sod tube that
 // raised the mixed form's L1(rho) distance to the FP64 form from 1e-7 to 2e-5.
 template <class Q> struct SampleOps;
 template <> struct SampleOps<double> {
@@ -583,22 +585,21 @@ template <> struct SampleOps<double> {
 };
 template <> struct SampleOps<float> : SampleOps<double> {
     // Rankine-Hugoniot (riemann.F90:22) in delta form: rho (g1 x + 1)/(g1 + x)
-    // = rho + rho (g1 - 1)(pm - p)/(g1 p + pm), the FP32 ratio scaling the
-    // jump only (exactly rho when pm = p)
+    // = rho + rho (g1 - 1)(pm - p)/(g1 p + pm), the approximate quotient
+    // scaling the jump only (exactly rho when pm = p)
     static __device__ __forceinline__ double shock_density(double g1, double pm, double p, double rho)
     {
-        const float num = (float)((g1 - 1.0) * (pm - p));
-        const float den = (float)(g1 * p + pm);
-        return rho + rho * (double)(num * __builtin_amdgcn_rcpf(den));
+        return rho + rho * ((g1 - 1.0) * ((pm - p) * __builtin_amdgcn_rcp(g1 * p + pm)));
     }
     static __device__ __forceinline__ double sound(double gamma, double p, double rho)
     {
-        return (double)__builtin_amdgcn_sqrtf((float)(gamma * p * __builtin_amdgcn_rcp(rho)));
+        const double c2 = gamma * p * __builtin_amdgcn_rcp(rho);
+        return c2 > 0.0 ? c2 * __builtin_amdgcn_rsq(c2) : 0.0;
     }
-    // riemann.F90:7: f2 + (f1 - f2) vt/(vt - vh), the weight in FP32
+    // riemann.F90:7: f2 + (f1 - f2) vt/(vt - vh), the weight approximate
     static __device__ __forceinline__ double fan(double f1, double f2, double vt, double vh)
     {
-        return f2 + (double)(float)(vt * __builtin_amdgcn_rcp(vt - vh)) * (f1 - f2);
+        return f2 + (vt * __builtin_amdgcn_rcp(vt - vh)) * (f1 - f2);
     }
 };
 
