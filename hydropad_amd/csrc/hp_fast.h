@@ -342,7 +342,12 @@ __device__ __forceinline__ Face<double> trace_face_d(const SweepScalars<R>& S, c
     if (f.p <= 0.0) f.p = ptl;
     f.u = a[VU] + (double)(ut + ct * (bp - bm));
     // 1/(1/rho_t - bm - b0 - bp) = rho_t / (1 - rho_t (bm + b0 + bp))
-    f.r = rtl * rcp(1.0 - (double)((R)rtl * (bm + b0 + bp)));
+    if constexpr (sizeof(R) == 8) {
+        f.r = rtl * rcp(1.0 - (double)((R)rtl * (bm + b0 + bp)));
+    } else {                                        // delta form: rtl + rtl s/(1 - s)
+        const R sb = (R)rtl * (bm + b0 + bp);
+        f.r = rtl + rtl * (double)(sb * rcp(R(1) - sb));
+    }
     if (f.r <= 0.0) f.r = rtl;
     f.v = a[VV] + (double)vf;
     f.w = a[VW] + (double)wf;
@@ -371,7 +376,16 @@ __device__ __forceinline__ void cell_faces_d(const SweepScalars<R>& S, double ga
         if (v == VR && steep_rho) steepen_d(rm2, rm1, a[VR], rp1, rp2, drm1, drp1, l, r);
         q[v] = limit_parab_d(l, r, flatv);
     }
-    const double cs = fsqrt(gamma * a[VP] * rcp(a[VR]));        // ppmsolver.F90:109
+    // ppmsolver.F90:109; the mixed form takes it at single-precision accuracy
+    // (raw v_rcp_f64 / v_rsq_f64): it sets the characteristic speeds u -+ c,
+    // whose signs only switch corrections that vanish where u -+ c does
+    double cs;
+    if constexpr (sizeof(R) == 8) {
+        cs = fsqrt(gamma * a[VP] * rcp(a[VR]));
+    } else {
+        const double c2 = gamma * a[VP] * __builtin_amdgcn_rcp(a[VR]);
+        cs = c2 > 0.0 ? c2 * __builtin_amdgcn_rsq(c2) : 0.0;
+    }
     fR = trace_face_d<R, GRAV, COSMO, true>(S, q, a, cs, gamma, nclR);
     fL = trace_face_d<R, GRAV, COSMO, false>(S, q, a, cs, gamma, nclL);
 }
