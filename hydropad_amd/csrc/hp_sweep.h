@@ -80,7 +80,7 @@ struct FastMath {
                                                       const double* ivr, double rm2, double rm1, double rp1,
                                                       double rp2, R drm1, R drp1, double pm1, double pp1, bool flat,
                                                       Face<double>& fL, Face<double>& fR, int& nclL, int& nclR);
-    // the mixed form (R = float) samples the Riemann solution in FP32
+    // the mixed form (R = float) samples the Riemann solution at single-precision accuracy
     template <class R>
     static __device__ __forceinline__ Flux riemann(const RScal& Rs, const Face<double>& L, const Face<double>& Rt,
                                                    int& nit)
