@@ -143,16 +143,21 @@ __device__ __forceinline__ float pow_pos(float x, float y)
 // error e to about 3e^2), then x^(1/5) = x y^4.  The seed needs x inside
 // FP32's normal range; near-vacuum rarefactions produce pressure ratios far
 // below it (p*/p ~ 1e-45 and less), so outside [1e-30, 1e30] the argument is
-// first scaled by 2^(-5q) and the root by 2^q.  x = 0 gives 0.
-__device__ __forceinline__ double root5(double x)
+// first scaled by 2^(-5q) and the root by 2^q.  x = 0 gives 0.  rx45
+// receives x^(-4/5) = x^(1/5)/x, the y^4 on the way (0 for x = 0).
+__device__ __forceinline__ double root5(double x, double& rx45)
 {
-    double xs = x, sc = 1.0;
+    double xs = x, sc = 1.0, sc4 = 1.0;
     if (!(x > 1e-30 && x < 1e30)) {                 // rare: wave-uniformly skipped
-        if (!(x > 0.0)) return 0.0;
+        if (!(x > 0.0)) {
+            rx45 = 0.0;
+            return 0.0;
+        }
         const int e = __builtin_amdgcn_frexp_exp(x);
         const int q = (e >= 0) ? e / 5 : -((4 - e) / 5);
         xs = __builtin_amdgcn_ldexp(x, -5 * q);
         sc = __builtin_amdgcn_ldexp(1.0, q);
+        sc4 = __builtin_amdgcn_ldexp(1.0, -4 * q);
     }
     double y = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)xs));
     double y2 = y * y;
@@ -160,7 +165,14 @@ __device__ __forceinline__ double root5(double x)
     y2 = y * y;
     y = y * fma(-0.2 * xs, y2 * y2 * y, 1.2);
     y2 = y * y;
-    return xs * (y2 * y2) * sc;
+    const double y4 = y2 * y2;
+    rx45 = y4 * sc4;
+    return xs * y4 * sc;
+}
+__device__ __forceinline__ double root5(double x)
+{
+    double rx45;
+    return root5(x, rx45);
 }
 
 // The three powers the Riemann solver takes: x^m, x^(1/gamma), x^(1/m).  For
@@ -420,13 +432,18 @@ __device__ __forceinline__ void rsq_sqrt(double x, double& rs, double& sq)
 // wave impedance 1/a and the left side's form of dv*/dp* (the right side's is
 // its negative).  shock = pstar >= p_side; cl2 = c^2, rc = 1/c for the side's
 // Lagrangian sound speed c; x = pstar/p_side, xinv = p_side/pstar.
-__device__ __forceinline__ void newton_side(const RScal& Rs, bool shock, double x, double xinv, double cl2,
-                                            double rc, double& ra, double& vp)
+__device__ __forceinline__ void newton_side(const RScal& Rs, bool shock, double x, double pstar, double pside,
+                                            double cl2, double rc, double& ra, double& vp)
 {
     const double m = Rs.m, gf = Rs.gf;
-    double w = 1.0;
+    double w = 1.0, wx = 0.0;                        // x^m and x^m / x
     if (!shock) {
-        w = pow_m(Rs, x);
+        if (Rs.m5) {
+            w = root5(x, wx);
+        } else {
+            w = pow_pos(x, m);
+            wx = w * (pside * rcp(pstar));
+        }
         shock = (w == 1.0);                          // pstup == 0: the shock impedance (:74-75)
     }
     if (shock) {                                     // a = c sqrt(gf x + m), v' = -(a^2 + c^2)/(2 a^3)
@@ -437,7 +454,7 @@ __device__ __forceinline__ void newton_side(const RScal& Rs, bool shock, double 
         vp = -(cl2 * q + cl2) * 0.5 * (ra * ra * ra);
     } else {                                         // a = m c (1 - x)/(1 - x^m), v' = -(1/c) x^m / x
         ra = (1.0 - w) * rcp(m * (1.0 - x)) * rc;
-        vp = -rc * w * xinv;
+        vp = -rc * wx;
     }
 }
 
@@ -466,10 +483,9 @@ __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vs
             break;
         }
         const double pstarold = pstar;
-        const double rps = rcp(pstar);
         double vpl, vpr;
-        newton_side(Rs, pstar >= pleft, pstar * rpl, pleft * rps, cl2, rcl, ra, vpl);
-        newton_side(Rs, pstar >= pright, pstar * rpr, pright * rps, cr2, rcr, rb, vpr);
+        newton_side(Rs, pstar >= pleft, pstar * rpl, pstar, pleft, cl2, rcl, ra, vpl);
+        newton_side(Rs, pstar >= pright, pstar * rpr, pstar, pright, cr2, rcr, rb, vpr);
         const double vstarl = vxleft - (pstar - pleft) * ra;
         const double vstarr = vxright + (pstar - pright) * rb;
         pstar = pstar - div(vstarl - vstarr, vpl + vpr);     // vprimel - vprimer = vpl + vpr
