@@ -488,7 +488,13 @@ __device__ __forceinline__ int newton(const RScal& Rs, double& pstar, double& vs
         newton_side(Rs, pstar >= pright, pstar * rpr, pstar, pright, cr2, rcr, rb, vpr);
         const double vstarl = vxleft - (pstar - pleft) * ra;
         const double vstarr = vxright + (pstar - pright) * rb;
-        pstar = pstar - div(vstarl - vstarr, vpl + vpr);     // vprimel - vprimer = vpl + vpr
+        // the step (vprimel - vprimer = vpl + vpr) with one Newton step on the
+        // reciprocal and no quotient correction: within 2^-47 of the step,
+        // which moves the final iterate by at most that much of its last step
+        const double d = vpl + vpr;
+        double rd = __builtin_amdgcn_rcp(d);
+        rd = fma(fma(-d, rd, 1.0), rd, rd);
+        pstar = pstar - (vstarl - vstarr) * rd;
         if (pstar <= 0.0) pstar = pstarold * 1e-3;
         if (!(fabs(pstarold - pstar) > K_TERR * pstarold)) break;
     }
