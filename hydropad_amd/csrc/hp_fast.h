@@ -66,6 +66,17 @@ __device__ __forceinline__ double fsqrt(double x)
 }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// 1/sqrt(x) and sqrt(x) for x > 0 together: v_rsq_f64 seed and two Newton
+// steps on 1/sqrt, y <- y (1.5 - 0.5 x y^2).
+__device__ __forceinline__ void rsq_sqrt(double x, double& rs, double& sq)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * (x * y), y, 1.5);
+    y = y * fma(-0.5 * (x * y), y, 1.5);
+    rs = y;
+    sq = x * y;
+}
+
 constexpr double LN2_HI = 6.93147180369123816490e-01;
 constexpr double LN2_LO = 1.90821492927058770002e-10;
 constexpr double INV_LN2 = 1.44269504088896338700e+00;
@@ -322,8 +333,15 @@ __device__ __forceinline__ Face<double> trace_face_d(const SweepScalars<R>& S, c
     double ptl = a[VP] + (double)pt, rtl = a[VR] + (double)rt;
     if (ptl < 0.0) { ptl = K_1EM15; pt = (R)(K_1EM15 - a[VP]); ++nclamp; }
     if (rtl < 0.0) { rtl = K_1EM15; rt = (R)(K_1EM15 - a[VR]); ++nclamp; }
-    const R ct = fsqrt((R)(gamma * ptl * rtl));
-    const R rct = rcp(ct);
+    // the Lagrangian sound speed and its inverse (order2interp.F90:192-193):
+    // in FP64 both from one rsq; in FP32 the hardware root and reciprocal
+    R ct, rct;
+    if constexpr (sizeof(R) == 8) {
+        rsq_sqrt(gamma * ptl * rtl, rct, ct);
+    } else {
+        ct = fsqrt((R)(gamma * ptl * rtl));
+        rct = rcp(ct);
+    }
     const R ap = (R)a[VP], au = (R)a[VU];
     R bm = R(0), bp = R(0), b0 = R(0);
     R vf = vt, wf = wt;
@@ -417,16 +435,6 @@ __device__ __forceinline__ bool pcheck_guess(const RScal& Rs, double pleft, doub
     return false;
 }
 
-// 1/sqrt(x) and sqrt(x) for x > 0 together: v_rsq_f64 seed and two Newton
-// steps on 1/sqrt, y <- y (1.5 - 0.5 x y^2).
-__device__ __forceinline__ void rsq_sqrt(double x, double& rs, double& sq)
-{
-    double y = __builtin_amdgcn_rsq(x);
-    y = y * fma(-0.5 * (x * y), y, 1.5);
-    y = y * fma(-0.5 * (x * y), y, 1.5);
-    rs = y;
-    sq = x * y;
-}
 
 // One side of the van Leer iteration at pstar (riemann.F90:70-97): the inverse
 // wave impedance 1/a and the left side's form of dv*/dp* (the right side's is
