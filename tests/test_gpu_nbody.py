@@ -3,7 +3,8 @@ particle term of hydropad/speed.F90 and their place in
 hydropad/evolve_sys.F90:63-84) through the C-ABI against the oracle, which
 is pinned bit for bit to the reference's own kick and drift.  The particle
 arithmetic has no transcendental functions and is built without FP
-contraction, so the bar is bit-for-bit."""
+contraction, so the bar is bit-for-bit (to 1e-13 in the comoving loop, whose
+schedule scalars the device and the oracle compute separately)."""
 import numpy as np
 import pytest
 
@@ -107,3 +108,52 @@ def test_evolve_with_particles_vs_oracle(oracle, precision):
     for k in FIELDS:
         assert rel_l1(out[k], ref[k]) <= 1e-12, k
     assert np.array_equal(xg, xo) and np.array_equal(vg, vo)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64_exact"])
+def test_evolve_cosmo_with_particles_and_gravity_vs_oracle(oracle, precision):
+    """The comoving loop with gravity and particles (hydropad/evolve_sys.F90
+    with COSMO, GRAVITY and NBODY on one rank): sweeps with the gravity and
+    expansion terms, kick, drift, kick with the step's dat and rdtath,
+    expand, the CFL maximum over cells and particles, timedet_cosmo.  The
+    device loop fuses the expansion into the last sweep and the particle
+    speeds into the maximum; the oracle runs the reference's order."""
+    print(f"nbody evolve_cosmo {precision}", flush=True)
+    N, K, dtat = 16, 8, 0.002
+    shape = (N + 2 * NB,) * 3
+    g = ic.gaussian_gforce(shape)
+    x, v = ic.particles((N, N, N), 2000, 17, vamp=0.05)
+    st = ic.grf_box(N, 5)
+    ca, tstop = ic.eds_schedule(1.0 / 51.0, 1.0 / 11.0, 200)
+    P = oracle.params(ngrid=N)
+    s = oracle.cosmo_init(1.0 / 51.0, 0.8, ca, tstop)
+    ref, xo, vo = copy_state(st), x.copy(), v.copy()
+    for nstep in range(1, K + 1):
+        P.at, P.dat, P.rat, P.dt, P.rdtath = s.at, s.dat, s.rat, s.dt, s.rdtath
+        old = {k: ref[k].copy(order="F") for k in ("p", "vx", "vy", "vz")}
+        assert oracle.ppm_lxlylz(P, ref, hp.nrot_of_step(nstep), gforce=g) >= 0
+        assert oracle.kick(P, xo, vo, g, dtat) == 0
+        oracle.drift(P, xo, vo, (N, N, N))
+        assert oracle.kick(P, xo, vo, g, dtat) == 0
+        oracle.expand(s.dt * (s.dath / s.ath), ref, old)
+        oracle.timedet_cosmo(s, max(oracle.speed(P, ref), oracle.particle_vmax(vo)), P.dx)
+    sg = hp.cosmo_init(1.0 / 51.0, 0.8, ca, tstop)
+    with hp.Box(shape, precision) as box:
+        box.upload(copy_state(st), gforce=g)
+        box.particles_upload(x, v, dtat)
+        box.evolve_cosmo(1, K, sg)
+        out = box.download()
+        xg, vg = box.particles_download()
+        c = box.counters()
+    assert sg.nstep == s.nstep == K
+    for name in ("t", "dt", "at", "dat", "ath", "dath", "rdtath"):
+        assert getattr(sg, name) == pytest.approx(getattr(s, name), rel=1e-14), name
+    for k in FIELDS:
+        assert rel_l1(out[k], ref[k]) <= 1e-12, k
+    # the particle arithmetic is bit-exact; the schedule's dat and rdtath,
+    # computed on the device and in the oracle, agree to ~1e-15
+    ex = np.abs(xg - xo).max() / np.abs(xo).max()
+    ev = np.abs(vg - vo).max() / np.abs(vo).max()
+    print(f"particles: max rel dx {ex:.2e} dv {ev:.2e}", flush=True)
+    assert ex <= 1e-13 and ev <= 1e-13
+    assert c.particles_outside == 0 and c.newton_nonconv == 0
