@@ -11,10 +11,14 @@
 //     exp via range-reduced Taylor) accurate to a few ulp, instead of
 //     OCML's correctly-rounded-pow machinery (~224 VALU ops per call);
 //   * the Newton iteration evaluates (p*/p)^m once per side and gets
-//     (p*/p)^(-gf) as (p*/p)^m / (p*/p), since -gf = m - 1;
+//     (p*/p)^(-gf) = (p*/p)^(m-1) from the same root (for gamma = 5/3 the
+//     fifth root's own y^4); its step quotient skips the residual
+//     correction, which only scales the last step;
 //   * the shock+rarefaction initial guess (three pow) is only formed when the
 //     reference would actually use it (riemann.F90:54-56 overrides it with
-//     pcheck whenever pcheck > 0).
+//     pcheck whenever pcheck > 0);
+//   * a contact at rest (equal pressures, no normal velocity) is returned
+//     exactly without iterating (the value the full path gives).
 // R = double is the FP64 form; R = float is the mixed form's reconstruction,
 // tracing and Riemann sampling, with the Newton pressure iteration and the
 // conservative + dual-energy update kept in FP64.
