@@ -787,8 +787,11 @@ __device__ __forceinline__ Flux riemann_solve(const RScal& Rs, const Face<double
     // stays one selection of values.
     RSolve S;
     if (L.p == Rt.p && L.u == 0.0 && Rt.u == 0.0) {
+        S.rrl = S.rrr = S.rcle = S.cleft = S.rcre = S.cright = 0.0;   // unused by pattern 0
         S.kase = 0;
         S.nit = 1;
+        S.pm = L.p;
+        S.vxm = 0.0;
     } else {
         riemann_setup(Rs, L, Rt, S);
         if (S.kase == 1 || S.kase == 2) riemann_newton(Rs, L, Rt, S);
