@@ -160,29 +160,35 @@ __device__ __forceinline__ float pow_pos(float x, float y)
 // below it (p*/p ~ 1e-45 and less), so outside [1e-30, 1e30] the argument is
 // first scaled by 2^(-5q) and the root by 2^q.  x = 0 gives 0.  rx45
 // receives x^(-4/5) = x^(1/5)/x, the y^4 on the way (0 for x = 0).
-__device__ __forceinline__ double root5(double x, double& rx45)
+// The common case runs unconditionally and the rare one overwrites its
+// results, so the in-range path carries no merge moves.
+__device__ __forceinline__ double root5_core(double xs, double& y4)
 {
-    double xs = x, sc = 1.0, sc4 = 1.0;
-    if (!(x > 1e-30 && x < 1e30)) {                 // rare: wave-uniformly skipped
-        if (!(x > 0.0)) {
-            rx45 = 0.0;
-            return 0.0;
-        }
-        const int e = __builtin_amdgcn_frexp_exp(x);
-        const int q = (e >= 0) ? e / 5 : -((4 - e) / 5);
-        xs = __builtin_amdgcn_ldexp(x, -5 * q);
-        sc = __builtin_amdgcn_ldexp(1.0, q);
-        sc4 = __builtin_amdgcn_ldexp(1.0, -4 * q);
-    }
     double y = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)xs));
     double y2 = y * y;
     y = y * fma(-0.2 * xs, y2 * y2 * y, 1.2);
     y2 = y * y;
     y = y * fma(-0.2 * xs, y2 * y2 * y, 1.2);
     y2 = y * y;
-    const double y4 = y2 * y2;
-    rx45 = y4 * sc4;
-    return xs * y4 * sc;
+    y4 = y2 * y2;
+    return xs * y4;
+}
+__device__ __forceinline__ double root5(double x, double& rx45)
+{
+    double r = root5_core(x, rx45);
+    if (!(x > 1e-30 && x < 1e30)) {                 // rare: wave-uniformly skipped
+        if (!(x > 0.0)) {
+            rx45 = 0.0;
+            r = 0.0;
+        } else {
+            const int e = __builtin_amdgcn_frexp_exp(x);
+            const int q = (e >= 0) ? e / 5 : -((4 - e) / 5);
+            double y4;
+            r = root5_core(__builtin_amdgcn_ldexp(x, -5 * q), y4) * __builtin_amdgcn_ldexp(1.0, q);
+            rx45 = y4 * __builtin_amdgcn_ldexp(1.0, -4 * q);
+        }
+    }
+    return r;
 }
 __device__ __forceinline__ double root5(double x)
 {
