@@ -326,17 +326,24 @@ __device__ __forceinline__ void expand_cell(const Epilogue& E, long q, double* v
     val[0] = pn;
 }
 
+// The epilogue of one updated cell before its store: expansion, CFL maximum.
+template <class M, bool EXP>
+__device__ __forceinline__ void settle_cell(const Epilogue& E, double gamma, long q, double* val, Tally& T)
+{
+    if (EXP) expand_cell(E, q, val);
+    if (E.do_speed) {
+        const double vs = M::vsound(gamma, val[0], val[1]);
+        T.vmax = tmax(T.vmax, tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs);
+    }
+}
+
 // The epilogue of one updated cell: expansion, store (+ ghost images), CFL maximum.
 template <class M, bool EXP>
 __device__ __forceinline__ void finish_cell(const BoxView& B, const Epilogue& E, double gamma, long q, double* val,
                                             int ix, int iy, int iz, Tally& T)
 {
-    if (EXP) expand_cell(E, q, val);
+    settle_cell<M, EXP>(E, gamma, q, val, T);
     store_cell(B.out, q, val, ix, iy, iz, E.ghost_mask, B.nx, B.ny, B.nz, B.nb);
-    if (E.do_speed) {
-        const double vs = M::vsound(gamma, val[0], val[1]);
-        T.vmax = tmax(T.vmax, tmax(tmax(fabs(val[2]), fabs(val[3])), fabs(val[4])) + vs);
-    }
 }
 
 // Wave-level reduction of the per-lane counters, then one atomic per wave.
@@ -610,6 +617,21 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
     fRprev.p = fRprev.r = fRprev.u = fRprev.v = fRprev.w = 0.0;
     Fprev.p = Fprev.r = Fprev.u = Fprev.v = Fprev.w = 0.0;
 
+    // The stores of the cell updated in one iteration are issued at the top of
+    // the next, before that iteration's prefetch: the wait for the prefetched
+    // values (vmcnt counts stores as well as loads) then finds both a whole
+    // iteration old instead of stalling on stores issued just before it.  The
+    // comoving kernels keep the store in place: the five values held across
+    // the iteration boundary would cost them occupancy or spills.
+    constexpr bool DEFER = !COSMO;
+    double pend[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int ixc = nb + ia, ibc = nb + ib;
+    auto store_pending = [&](int kc) {                 // cell kc of the segment, updated last iteration
+        const int ic = nb + kc;
+        if (DIR == 2) store_cell(B.out, base + (long)kc * s_ax, pend, ixc, ic, ibc, E.ghost_mask, B.nx, B.ny, B.nz, nb);
+        else store_cell(B.out, base + (long)kc * s_ax, pend, ixc, ibc, ic, E.ghost_mask, B.nx, B.ny, B.nz, nb);
+    };
+
     if (active) {
         double nxt[NV];
         {
@@ -623,6 +645,7 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
             double cur[NV];
 #pragma unroll
             for (int s = 0; s < NV; ++s) cur[s] = nxt[s];
+            if (DEFER && kk >= s0 + 4) store_pending(kk - 4);
             if (kk + 1 <= s1 + 2) {                    // prefetch the next cell of the pencil
                 const long qn = q + s_ax;
                 nxt[VP] = B.in[0][qn]; nxt[VR] = B.in[1][qn];
@@ -671,13 +694,9 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
                 double pres = w1[VP], rho = w1[VR], u = w1[VU], vv = w1[VV], w = w1[VW];
                 const double etot = total_phys<DIR>(A.rgamma1, pres, rho, u, vv, w);
                 M::integrate(A.Is, Fprev, F, pres, rho, u, vv, w, etot, GRAV ? w1[NV - 1] : 0.0, GRAV);
-                double val[5];
-                val[0] = pres; val[1] = rho; val[D::U] = u; val[D::V] = vv; val[D::W] = w;
-                const int ic = nb + kk - 3;
-                int ix = nb + ia, iy, iz;
-                if (DIR == 2) { iy = ic; iz = nb + ib; }
-                else { iy = nb + ib; iz = ic; }
-                finish_cell<M, EXP>(B, E, A.gamma, q - 3 * s_ax, val, ix, iy, iz, T);
+                pend[0] = pres; pend[1] = rho; pend[D::U] = u; pend[D::V] = vv; pend[D::W] = w;
+                settle_cell<M, EXP>(E, A.gamma, q - 3 * s_ax, pend, T);
+                if (!DEFER) store_pending(kk - 3);
             }
             // slide the window
             r5 = w0[VR];
@@ -690,6 +709,7 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
             fRprev = fR;
             Fprev = F;
         }
+        if (DEFER && s1 + 2 >= s0 + 3) store_pending(s1 - 1);   // the last cell, updated at kk = s1 + 2
     }
 }
 
