@@ -543,6 +543,14 @@ __device__ __forceinline__ void x_run(const BoxView& B, const AllScalars<R>& A, 
         const int nclRm1 = from_left(nclR), nclLp1 = from_right(nclL), nitp1 = from_right(nit);
 
         const bool valid = in_range && lane >= 3 && lane <= 60 && i >= nb && i < nb + Nx;
+        // Take delivery of the next chunk's prefetch (issued a whole chunk ago)
+        // here, before this chunk's stores: vmcnt counts stores as well as
+        // loads, so a first use at the top of the next chunk would also wait
+        // for these stores to complete.  The empty asm is that use.
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int s = 0; s < NV; ++s) asm volatile("" : "+v"(nxt[s]));
+        }
         if (valid) {
             count_newton(nit, T.nitmax, T.nonconv);
             T.nclamp += nclRm1 + nclL;
@@ -621,8 +629,9 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
     // the next, before that iteration's prefetch: the wait for the prefetched
     // values (vmcnt counts stores as well as loads) then finds both a whole
     // iteration old instead of stalling on stores issued just before it.  The
-    // comoving kernels keep the store in place: the five values held across
-    // the iteration boundary would cost them occupancy or spills.
+    // comoving kernels keep the store in place (the five values held across
+    // the iteration boundary would cost them occupancy or spills) and take
+    // delivery of the prefetch before it instead.
     constexpr bool DEFER = !COSMO;
     double pend[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int ixc = nb + ia, ibc = nb + ib;
@@ -639,6 +648,10 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
             nxt[VP] = B.in[0][q0]; nxt[VR] = B.in[1][q0];
             nxt[VU] = B.in[D::U][q0]; nxt[VV] = B.in[D::V][q0]; nxt[VW] = B.in[D::W][q0];
             if (GRAV) nxt[NV - 1] = -B.gforce[3 * q0 + (DIR - 1)];
+        }
+        if constexpr (!DEFER) {     // the first cell too, so the loop head holds no wait
+#pragma unroll
+            for (int s = 0; s < NV; ++s) asm volatile("" : "+v"(nxt[s]));
         }
         for (int kk = s0 - 3; kk <= s1 + 2; ++kk) {
             const long q = base + (long)kk * s_ax;
@@ -689,6 +702,12 @@ __device__ __forceinline__ void stream_run(const BoxView& B, const AllScalars<R>
                 // the reference's loop over interfaces: a segment owns
                 // interfaces s0 .. s1-1 and the pencil's last segment also Nax
                 if (kk != s1 + 2 || s1 == Nax) count_newton(nit, T.nitmax, T.nonconv);
+            }
+            if constexpr (!DEFER) {
+                // take delivery of the prefetch before this cell's stores
+                // (the empty asm is the use; see x_run)
+#pragma unroll
+                for (int s = 0; s < NV; ++s) asm volatile("" : "+v"(nxt[s]));
             }
             if (kk >= s0 + 3) {     // update cell k-3
                 double pres = w1[VP], rho = w1[VR], u = w1[VU], vv = w1[VV], w = w1[VW];
