@@ -253,33 +253,32 @@ __device__ __forceinline__ void FastMath::cell_faces(const AllScalars<R>& A, con
 
 // Store one updated cell and, for the axes in `mask`, its periodic face-ghost
 // image (ghost g <- cell g+N on the low side, ghost N+nb+g <- cell nb+g).
+// Each image is its own conditional store: only cells within 2 nb of a face
+// have one, so a wave almost always skips them (a list of shifts built per
+// cell instead cost every cell some sixty instructions of bookkeeping).
 __device__ __forceinline__ void store_cell(double* const* out, long q, const double* val, int ix, int iy, int iz,
                                            int mask, int nx, int ny, int nz, int nb)
 {
 #pragma unroll
     for (int f = 0; f < 5; ++f) out[f][q] = val[f];
-    if (mask) {
-        long shift[6];
-        int ns = 0;
-        if (mask & 1) {
-            const int N = nx - 2 * nb;
-            if (ix < 2 * nb) shift[ns++] = N;
-            if (ix >= N) shift[ns++] = -N;
-        }
-        if (mask & 2) {
-            const long N = ny - 2 * nb;
-            if (iy < 2 * nb) shift[ns++] = N * nx;
-            if (iy >= N) shift[ns++] = -N * nx;
-        }
-        if (mask & 4) {
-            const long N = nz - 2 * nb;
-            if (iz < 2 * nb) shift[ns++] = N * nx * (long)ny;
-            if (iz >= N) shift[ns++] = -N * nx * (long)ny;
-        }
-        for (int s = 0; s < ns; ++s) {
+    auto image = [&](long shift) {
 #pragma unroll
-            for (int f = 0; f < 5; ++f) out[f][q + shift[s]] = val[f];
-        }
+        for (int f = 0; f < 5; ++f) out[f][q + shift] = val[f];
+    };
+    if (mask & 1) {
+        const int N = nx - 2 * nb;
+        if (ix < 2 * nb) image(N);
+        if (ix >= N) image(-N);
+    }
+    if (mask & 2) {
+        const long N = ny - 2 * nb;
+        if (iy < 2 * nb) image(N * nx);
+        if (iy >= N) image(-N * nx);
+    }
+    if (mask & 4) {
+        const long N = nz - 2 * nb;
+        if (iz < 2 * nb) image(N * nx * (long)ny);
+        if (iz >= N) image(-N * nx * (long)ny);
     }
 }
 
