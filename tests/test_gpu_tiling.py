@@ -61,3 +61,36 @@ def test_chunk_edge_and_minimum_shapes_vs_oracle(oracle, shape):
         out = sweep(st, P, d, "fp64")
         for k in FIELDS:
             assert rel_l1(out[k], ref[k]) <= 1e-12, (shape, d, k, rel_l1(out[k], ref[k]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp64_exact"])
+@pytest.mark.parametrize("shape", [(3, 3, 3), (4, 5, 3), (12, 7, 9), (53, 6, 4)],
+                         ids=lambda s: "x".join(map(str, s)) if isinstance(s, tuple) else s)
+def test_sweeps_write_the_periodic_face_ghosts(precision, shape):
+    """Every sweep writes, beside each updated cell, its periodic image in the
+    face ghosts of all three axes (what exchange_mesh would copy): after one
+    sweep in each direction and under two launch shapes, each face-ghost slab
+    equals the interior slab it images, bit for bit.  With an interior of 3
+    cells (< 2 nb) every cell has both images along that axis."""
+    st = random_box(shape, 41 + sum(shape))
+    N = [n for n in shape]
+    sl = [slice(NB, NB + n) for n in N]
+    for d in (1, 2, 3):
+        for tiling in (None, (1, 1, 64)):
+            with hp.Box(st["p"].shape, precision) as box:
+                if tiling is not None:
+                    box.set_tiling(*tiling)
+                box.upload(copy_state(st))
+                box.ppmsolver(d)
+                out = box.download()
+            for k in FIELDS:
+                a = out[k]
+                for ax in range(3):
+                    n = N[ax]
+                    face = list(sl)
+                    lo_g, lo_src = list(face), list(face)
+                    lo_g[ax], lo_src[ax] = slice(0, NB), slice(n, n + NB)
+                    hi_g, hi_src = list(face), list(face)
+                    hi_g[ax], hi_src[ax] = slice(n + NB, n + 2 * NB), slice(NB, 2 * NB)
+                    assert np.array_equal(a[tuple(lo_g)], a[tuple(lo_src)]), (precision, shape, d, tiling, k, ax, "low")
+                    assert np.array_equal(a[tuple(hi_g)], a[tuple(hi_src)]), (precision, shape, d, tiling, k, ax, "high")
