@@ -452,9 +452,11 @@ __device__ __forceinline__ void x_run(const BoxView& B, const AllScalars<R>& A, 
     };
 
     // The FP64 form prefetches the next chunk into registers; the mixed form
-    // does not: its ten fewer registers give four waves per SIMD instead of
-    // three, which hides the loads better than the prefetch does.
-    constexpr bool PREFETCH = std::is_same<R, double>::value;
+    // does where the ten registers still leave four waves per SIMD (the
+    // gamma = 5/3 kernel without gravity or expansion terms, 125 VGPRs);
+    // elsewhere its ten fewer registers give it a wave more per SIMD, which
+    // hides the loads better than the prefetch does.
+    constexpr bool PREFETCH = std::is_same<R, double>::value || (M::G53 && !GRAV && !COSMO);
     long qn = 0;
     int in_ = 0, jn = 0, kn = 0;
     bool rn = false;
